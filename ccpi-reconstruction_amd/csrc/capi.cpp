@@ -1,0 +1,191 @@
+// C ABI of libccpi_hip.so (declared in include/ccpi_hip.h).  Every entry
+// point converts exceptions to status codes; nothing C++ crosses it.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "ccpi_hip.h"
+#include "common.hpp"
+#include "plan.hpp"
+#include "solvers.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    g_last_error.clear();
+    return CCPI_OK;
+  } catch (const ccpi::Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return CCPI_E_NOMEM;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return CCPI_E_HIP;
+  }
+}
+
+hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
+
+}  // namespace
+
+extern "C" {
+
+int ccpi_pb_f2D(const double *h_pixels, const double *v_pixels, const double *angles,
+                int n_angles, int n_h, int n_v, const double vox_origin[3],
+                const double vox_size[3], int nx, int ny, int nz, float *pixels,
+                const float *voxels, void *stream) {
+  return guarded([&] {
+    CCPI_REQUIRE(pixels && voxels, CCPI_E_ARG, "null data pointer");
+    const ccpi::Geometry G = ccpi::Geometry::parallel(h_pixels, v_pixels, angles, n_angles,
+                                                      n_h, n_v, vox_origin, vox_size, nx, ny, nz);
+    ccpi::Projector P(G, S(stream));
+    P.fp(voxels, pixels, S(stream));
+  });
+}
+
+int ccpi_pb_b2D(const double *h_pixels, const double *v_pixels, const double *angles,
+                const float *pixels, float *voxels, int n_angles, int n_h, int n_v,
+                const double vox_origin[3], const double vox_size[3], int nx, int ny, int nz,
+                void *stream) {
+  return guarded([&] {
+    CCPI_REQUIRE(pixels && voxels, CCPI_E_ARG, "null data pointer");
+    const ccpi::Geometry G = ccpi::Geometry::parallel(h_pixels, v_pixels, angles, n_angles,
+                                                      n_h, n_v, vox_origin, vox_size, nx, ny, nz);
+    ccpi::Projector P(G, S(stream));
+    P.bp(pixels, voxels, S(stream));
+  });
+}
+
+int ccpi_cb_f2D(double source_x, double source_y, double source_z, double detector_x,
+                const double *h_pixels, const double *v_pixels, const double *angles,
+                float *pixels, const float *voxels, int n_angles, int n_h, int n_v,
+                const double vox_origin[3], const double vox_size[3], int nx, int ny, int nz,
+                void *stream) {
+  return guarded([&] {
+    CCPI_REQUIRE(pixels && voxels, CCPI_E_ARG, "null data pointer");
+    const ccpi::Geometry G =
+        ccpi::Geometry::cone(source_x, source_y, source_z, detector_x, h_pixels, v_pixels,
+                             angles, n_angles, n_h, n_v, vox_origin, vox_size, nx, ny, nz);
+    ccpi::Projector P(G, S(stream));
+    P.fp(voxels, pixels, S(stream));
+  });
+}
+
+int ccpi_cb_b2D(double source_x, double source_y, double source_z, double detector_x,
+                const double *h_pixels, const double *v_pixels, const double *angles,
+                const float *pixels, float *voxels, int n_angles, int n_h, int n_v,
+                const double vox_origin[3], const double vox_size[3], int nx, int ny, int nz,
+                void *stream) {
+  return guarded([&] {
+    CCPI_REQUIRE(pixels && voxels, CCPI_E_ARG, "null data pointer");
+    const ccpi::Geometry G =
+        ccpi::Geometry::cone(source_x, source_y, source_z, detector_x, h_pixels, v_pixels,
+                             angles, n_angles, n_h, n_v, vox_origin, vox_size, nx, ny, nz);
+    ccpi::Projector P(G, S(stream));
+    P.bp(pixels, voxels, S(stream));
+  });
+}
+
+int ccpi_forward_project(const ccpi_geometry *g, float *pixels, const float *voxels,
+                         void *stream) {
+  return guarded([&] {
+    CCPI_REQUIRE(pixels && voxels, CCPI_E_ARG, "null data pointer");
+    ccpi::Projector P(ccpi::Geometry::from(g), S(stream));
+    P.fp(voxels, pixels, S(stream));
+  });
+}
+
+int ccpi_backward_project(const ccpi_geometry *g, const float *pixels, float *voxels,
+                          void *stream) {
+  return guarded([&] {
+    CCPI_REQUIRE(pixels && voxels, CCPI_E_ARG, "null data pointer");
+    ccpi::Projector P(ccpi::Geometry::from(g), S(stream));
+    P.bp(pixels, voxels, S(stream));
+  });
+}
+
+int ccpi_cgls(const ccpi_geometry *g, float *b, float *x, int iterations, double *norms,
+              int flags, void *stream) {
+  return guarded([&] {
+    CCPI_REQUIRE(b && x, CCPI_E_ARG, "null data pointer");
+    CCPI_REQUIRE(iterations >= 0, CCPI_E_ARG, "negative iteration count");
+    ccpi::Projector P(ccpi::Geometry::from(g), S(stream));
+    ccpi::Workspace W;
+    ccpi::run_cgls(P, b, x, iterations, norms, (flags & CCPI_REDUCE_FP32) != 0, W,
+                   S(stream));
+    CCPI_HIP(hipStreamSynchronize(S(stream)));
+  });
+}
+
+int ccpi_sirt(const ccpi_geometry *g, const float *b, float *x, int iterations, int flags,
+              void *stream) {
+  (void)flags;
+  return guarded([&] {
+    CCPI_REQUIRE(b && x, CCPI_E_ARG, "null data pointer");
+    CCPI_REQUIRE(iterations >= 0, CCPI_E_ARG, "negative iteration count");
+    ccpi::Projector P(ccpi::Geometry::from(g), S(stream));
+    ccpi::Workspace W;
+    ccpi::run_sirt(P, b, x, iterations, W, S(stream));
+    CCPI_HIP(hipStreamSynchronize(S(stream)));
+  });
+}
+
+int ccpi_mlem(const ccpi_geometry *g, const float *b, float *x, int iterations, int flags,
+              void *stream) {
+  (void)flags;
+  return guarded([&] {
+    CCPI_REQUIRE(b && x, CCPI_E_ARG, "null data pointer");
+    CCPI_REQUIRE(iterations >= 0, CCPI_E_ARG, "negative iteration count");
+    ccpi::Projector P(ccpi::Geometry::from(g), S(stream));
+    ccpi::Workspace W;
+    ccpi::run_mlem(P, b, x, iterations, W, S(stream));
+    CCPI_HIP(hipStreamSynchronize(S(stream)));
+  });
+}
+
+int ccpi_pb_cgls_batched(const ccpi_geometry *geoms, int count, float *const *b,
+                         float *const *x, int iterations, double *norms, int flags,
+                         void *stream) {
+  return guarded([&] {
+    CCPI_REQUIRE(geoms && b && x && count >= 0, CCPI_E_ARG, "bad batch arguments");
+    for (int i = 0; i < count; i++) {
+      CCPI_REQUIRE(geoms[i].kind == CCPI_PARALLEL, CCPI_E_ARG,
+                   "batched CGLS takes parallel-beam problems");
+      ccpi::Projector P(ccpi::Geometry::from(&geoms[i]), S(stream));
+      ccpi::Workspace W;
+      ccpi::run_cgls(P, b[i], x[i], iterations, norms ? norms + (size_t)i * iterations : nullptr,
+                     (flags & CCPI_REDUCE_FP32) != 0, W, S(stream));
+    }
+    CCPI_HIP(hipStreamSynchronize(S(stream)));
+  });
+}
+
+const char *ccpi_error_string(int status) {
+  switch (status) {
+    case CCPI_OK: return "ok";
+    case CCPI_E_ARG: return "invalid argument";
+    case CCPI_E_GEOMETRY: return "geometry not representable by the projector model";
+    case CCPI_E_HIP: return "HIP runtime error";
+    case CCPI_E_NOMEM: return "device allocation failed";
+    case CCPI_E_LIMIT: return "size beyond a kernel limit";
+    default: return "unknown status";
+  }
+}
+
+const char *ccpi_last_error(void) { return g_last_error.c_str(); }
+
+int ccpi_version(void) { return 100; }
+
+void ccpi_plan_cache_clear(void) { ccpi::plan_cache_clear(); }
+
+long long ccpi_plan_cache_bytes(void) { return ccpi::plan_cache_bytes(); }
+
+}  // extern "C"

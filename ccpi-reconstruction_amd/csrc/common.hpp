@@ -1,0 +1,93 @@
+// Shared definitions of the MI355X projector library: status handling,
+// plan layouts shared between host launchers and gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "ccpi_hip.h"
+
+namespace ccpi {
+
+// Carries a CCPI_E_* status through host code; converted to a return code
+// at the C ABI so no exception crosses it.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define CCPI_HIP(call)                                                      \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      throw ::ccpi::Error(e_ == hipErrorOutOfMemory ? CCPI_E_NOMEM          \
+                                                    : CCPI_E_HIP,           \
+                          std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define CCPI_REQUIRE(cond, code, msg)                                       \
+  do {                                                                      \
+    if (!(cond)) throw ::ccpi::Error((code), (msg));                        \
+  } while (0)
+
+// Model tolerance of every branch decision (p2D.cpp:20, c2D.cpp:15).
+constexpr double kEps = 1.1920928955078125e-07;  // FLT_EPSILON
+
+// ---- cone beam plan layout ------------------------------------------------
+
+// One source->pixel ray of the xy walk.  Entry m of the ray (m = 0..n-1)
+// holds the walk parameter alpha_m and, for m >= 1, the xy cell whose
+// segment is [alpha_{m-1}, alpha_m] (c2D.cpp:193-467 layout).
+struct alignas(32) ConeRay {
+  uint64_t off;   // first entry in the entry table
+  int n;          // number of entries (0: ray misses / outside footprint)
+  int ncom;       // first segment where an edge row can leave the volume
+  int min_v;      // lowest row still inside at the start of the last segment
+  int max_v;      // highest such row
+  int pad[2];
+};
+
+struct ConeEnt {
+  uint32_t col;   // i * ny + j
+  float alpha;
+};
+
+// Kernel argument block of the cone projectors (by value).
+struct ConeArgs {
+  int na, nh, nv, nx, ny, nz;
+  int mid;               // first detector row above the source
+  float pzbz, pzbz1, inv_dz;
+  float pzdv, z1, znm;   // BP safe-row bounds (c2D.cpp:993-1000)
+  double vx, vy, dx, dy; // volume xy origin and voxel size
+  double sx, sy, det_x, L;
+  double ipix, hpix0;    // detector column index map
+  const double *atab;    // [na][8]: c, s, p1x, p1y, c*det_x, s*det_x, L/c, L/s
+  const double *h_pix;   // [nh]
+  const int *hrange;     // [na][2] footprint of the whole volume
+  const float *dz;       // [nv] v_pix - source_z
+  const float *iz;       // [nv] 1 / dz
+  const float *vz;       // [nz+1] slice planes - source_z
+  const float *dconv;    // [nh][nv] source -> pixel distance
+};
+
+// ---- parallel beam plan layout ---------------------------------------------
+
+struct ParArgs {
+  int na, nh, nv, nx, ny, nz;
+  int map_type;          // 1, 2, 4 (rows per slice) or 0 (general)
+  double ox, oy, dx, dy; // volume xy origin and voxel size
+  double D;              // detector line distance 2 * max|origin|
+  double dconv;          // float(3 D) as double: walk start bound
+  double ihp, hp0;       // detector column index map
+  const double *atab;    // [na][5]: c, s, y_offset, 1/y_offset, length
+  const double *h_pix;   // [nh]
+  const int *map;        // [nv] row -> slice
+  const int *rlo;        // [nz] first row of slice k
+  const int *rhi;        // [nz] one past its last row
+};
+
+}  // namespace ccpi

@@ -1,0 +1,644 @@
+// Cone-beam forward projector and matched backprojector for gfx950.
+//
+// Discrete model: CCPi::cone_beam::f2D / b2D (Core/src/c2D.cpp:17-1119).
+//   * xy: Siddon walk of the source->pixel line in cell units, shared by
+//     every detector row of one (angle, h) column (c2D.cpp:193-467);
+//   * z: per row and xy segment, the entry slice k and at most one slice
+//     crossing split the segment between k and k-/+1 (c2D.cpp:17-191);
+//   * output scaled by the source->pixel distance d_conv (c2D.cpp:616-623).
+// Weights are computed with the reference's arithmetic (double geometry,
+// float alpha, no contraction: the library is built -ffp-contract=off), so
+// weights agree bit for bit with the model and only summation order
+// differs.
+//
+// Mapping to CDNA4:
+//   plan     one lane per (angle, h) ray walks the xy grid in double and
+//            writes the ray's (cell, alpha) list to HBM once per geometry.
+//   FP       one workgroup per ray column and 256-row block; lanes are
+//            detector rows, so the xy walk is uniform (scalar loads of the
+//            entry list) and each segment's voxel column reads are
+//            near-contiguous in z across the wave.
+//   BP       one wavefront per voxel column: lanes find the column's
+//            (angle, h) rays in parallel (double), compact them into LDS,
+//            then sweep rows and accumulate into an LDS copy of the column
+//            with ds_add_f32.  One wave owns a column, so the BP needs no
+//            global atomics and is deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "plan.hpp"
+
+namespace ccpi {
+
+// ---------------------------------------------------------------------------
+// plan: xy walk of every ray
+// ---------------------------------------------------------------------------
+
+// Walk state for one ray; FILL writes entries and tracks ncom.
+template <bool FILL>
+struct WalkSink {
+  ConeEnt *out;
+  int n = 0;
+  // ncom bookkeeping (c2D.cpp:49-78)
+  float prev = 0.f;
+  int min_xy = -1, max_xy = -1;
+  float pzbz, inv_dz, dz0, dzn;
+  int nzm1;
+
+  __device__ void emit(uint32_t col, float a) {
+    if (FILL) {
+      out[n] = ConeEnt{col, a};
+      if (n >= 1) {
+        const float ai = prev * inv_dz;
+        if (min_xy < 0) {
+          const int k = (int)floorf(pzbz + ai * dz0);
+          if (k <= 0) min_xy = (k == 0) ? n : 0;
+        }
+        if (max_xy < 0) {
+          const int k = (int)floorf(pzbz + ai * dzn);
+          if (k >= nzm1) max_xy = (k == nzm1) ? n : 0;
+        }
+      }
+      prev = a;
+    }
+    ++n;
+  }
+};
+
+// Mirrors the cone xy walk (c2D.cpp:227-457) with one unified step rule
+// for the four direction cases.
+template <class Sink>
+__device__ void cone_walk(double q1x, double q1y, double q2x, double q2y,
+                          int nx, int ny, Sink &sk) {
+  const double ddx = q2x - q1x, ddy = q2y - q1y;
+  const double ix = 1.0 / ddx, iy = 1.0 / ddy;
+  if (fabs(ddx) < kEps) {
+    if (fabs(ddy) >= kEps) {
+      const int i = (int)floor(q1x);
+      if (i >= 0 && i < nx) {
+        if (ddy < 0.0) {
+          sk.emit((uint32_t)(i * ny + ny - 1), (float)(((double)ny - q1y) * iy));
+          for (int j = ny - 1; j >= 0; j--)
+            sk.emit((uint32_t)(i * ny + j), (float)(((double)j - q1y) * iy));
+        } else {
+          sk.emit((uint32_t)(i * ny), (float)((0.0 - q1y) * iy));
+          for (int j = 0; j < ny; j++)
+            sk.emit((uint32_t)(i * ny + j), (float)(((double)(j + 1) - q1y) * iy));
+        }
+      }
+    }
+    return;
+  }
+  if (fabs(ddy) < kEps) {
+    const int j = (int)floor(q1y);
+    if (j >= 0 && j < ny) {
+      if (ddx < 0.0) {
+        sk.emit((uint32_t)((nx - 1) * ny + j), (float)(((double)nx - q1x) * ix));
+        for (int i = nx - 1; i >= 0; i--)
+          sk.emit((uint32_t)(i * ny + j), (float)(((double)i - q1x) * ix));
+      } else {
+        sk.emit((uint32_t)j, (float)((0.0 - q1x) * ix));
+        for (int i = 0; i < nx; i++)
+          sk.emit((uint32_t)(i * ny + j), (float)(((double)(i + 1) - q1x) * ix));
+      }
+    }
+    return;
+  }
+  const double ax0 = (0.0 - q1x) * ix, ay0 = (0.0 - q1y) * iy;
+  const double axn = ((double)nx - q1x) * ix, ayn = ((double)ny - q1y) * iy;
+  const double amin = fmax(fmax(fmin(ax0, axn), fmin(ay0, ayn)), 0.0);
+  const double amax = fmin(fmin(fmax(ax0, axn), fmax(ay0, ayn)), 1.0);
+  if (!(amin < amax - kEps)) return;
+  const int sx = ddx > 0.0 ? 1 : -1, sy = ddy > 0.0 ? 1 : -1;
+  int x, y;
+  {
+    const double xface = sx > 0 ? ax0 : axn, yface = sy > 0 ? ay0 : ayn;
+    const int xin = sx > 0 ? 0 : nx - 1, yin = sy > 0 ? 0 : ny - 1;
+    bool on_x = amin == xface, on_y = amin == yface;
+    if (sx > 0 && sy < 0) {
+      if (on_y) on_x = false;
+    } else if (!(sx < 0 && sy < 0)) {
+      if (on_x) on_y = false;
+    }
+    if (on_x) {
+      x = xin;
+      y = on_y ? yin : (int)floor(q1y + amin * ddy);
+    } else if (on_y) {
+      y = yin;
+      x = (int)floor(q1x + amin * ddx);
+    } else {
+      x = 0;
+      y = 0;
+    }
+    x = min(max(x, 0), nx - 1);
+    y = min(max(y, 0), ny - 1);
+  }
+  sk.emit((uint32_t)(x * ny + y), (float)amin);
+  const int bxs = sx > 0, bys = sy > 0;
+  while (x >= 0 && x < nx && y >= 0 && y < ny) {
+    const double nxa = ((double)(x + bxs) - q1x) * ix;
+    const double nya = ((double)(y + bys) - q1y) * iy;
+    const uint32_t col = (uint32_t)(x * ny + y);
+    if (nxa < nya - kEps) {
+      sk.emit(col, (float)nxa);
+      x += sx;
+    } else if (nya < nxa - kEps) {
+      sk.emit(col, (float)nya);
+      y += sy;
+    } else {
+      sk.emit(col, (float)fmax(nxa, nya));
+      x += sx;
+      y += sy;
+    }
+  }
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void cone_walk_kernel(
+    ConeArgs g, const uint64_t *__restrict__ offs, uint32_t *__restrict__ counts,
+    ConeRay *__restrict__ rays, ConeEnt *__restrict__ ents) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.na * g.nh) return;
+  const int a = r / g.nh, h = r - a * g.nh;
+  const int hlo = g.hrange[2 * a], hhi = g.hrange[2 * a + 1];
+  WalkSink<FILL> sk;
+  sk.out = FILL ? ents + offs[r] : nullptr;
+  sk.pzbz = g.pzbz;
+  sk.inv_dz = g.inv_dz;
+  sk.dz0 = g.dz[0];
+  sk.dzn = g.dz[g.nv - 1];
+  sk.nzm1 = g.nz - 1;
+  if (h >= hlo && h <= hhi) {
+    const double *t = g.atab + 8 * a;
+    const double c = t[0], s = t[1], p1x = t[2], p1y = t[3];
+    const double p2x = c * g.det_x - s * g.h_pix[h];
+    const double p2y = s * g.det_x + c * g.h_pix[h];
+    cone_walk((p1x - g.vx) / g.dx, (p1y - g.vy) / g.dy, (p2x - g.vx) / g.dx,
+              (p2y - g.vy) / g.dy, g.nx, g.ny, sk);
+  }
+  if (!FILL) {
+    counts[r] = (uint32_t)sk.n;
+    return;
+  }
+  ConeRay R;
+  R.off = offs[r];
+  R.n = sk.n;
+  R.ncom = 1;
+  R.min_v = g.mid;
+  R.max_v = g.mid - 1;
+  R.pad[0] = R.pad[1] = 0;
+  if (sk.n >= 2) {
+    const int n = sk.n;
+    int mx = sk.min_xy < 0 ? n : sk.min_xy;
+    int Mx = sk.max_xy < 0 ? n : sk.max_xy;
+    R.ncom = max(min(mx, Mx), 1);
+    // rows inside through the start of the last segment (c2D.cpp:80-95);
+    // the slice index is non-decreasing in v, so bisect.
+    const float ai = ents[R.off + n - 2].alpha * g.inv_dz;
+    int lo = 0, hi = g.mid;  // first v < mid with floor(...) > 0
+    while (lo < hi) {
+      const int m = (lo + hi) >> 1;
+      if ((int)floorf(g.pzbz + ai * g.dz[m]) > 0) hi = m; else lo = m + 1;
+    }
+    R.min_v = lo;
+    lo = g.mid;
+    hi = g.nv;  // first v >= mid with floor(...) >= nz - 1
+    while (lo < hi) {
+      const int m = (lo + hi) >> 1;
+      if ((int)floorf(g.pzbz + ai * g.dz[m]) >= g.nz - 1) hi = m; else lo = m + 1;
+    }
+    R.max_v = lo - 1;
+  }
+  rays[r] = R;
+}
+
+// ---------------------------------------------------------------------------
+// forward projection
+// ---------------------------------------------------------------------------
+
+constexpr int kFpRows = 256;
+
+__global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
+    ConeArgs g, const ConeRay *__restrict__ rays, const ConeEnt *__restrict__ ents,
+    const float *__restrict__ vol, float *__restrict__ sino) {
+  extern __shared__ float s_vz[];
+  for (int q = threadIdx.x; q <= g.nz; q += kFpRows) s_vz[q] = g.vz[q];
+  __syncthreads();
+  const int r = blockIdx.x;
+  const int v = blockIdx.y * kFpRows + threadIdx.x;
+  if (v >= g.nv) return;
+  const int h = r % g.nh;
+  const ConeRay R = rays[r];
+  float *out = sino + (size_t)r * g.nv + v;
+  float acc = *out;
+  if (R.n > 1) {
+    const float dzv = g.dz[v], izv = g.iz[v];
+    const bool lower = v < g.mid;
+    const int dir = lower ? -1 : 1;
+    const int nz = g.nz;
+    const ConeEnt *__restrict__ e = ents + R.off;
+    float a0 = e[0].alpha;
+    for (int m = 1; m < R.n; ++m) {
+      const ConeEnt en = e[m];
+      const float a1 = en.alpha;
+      const float ai = a0 * g.inv_dz;
+      // rows leaving through the bottom face use the floor-safe form
+      // (c2D.cpp:153-167); all other rows c2D.cpp:104-143, 175-188
+      const bool edge = (m >= R.ncom) && (v < R.min_v);
+      const int k = (int)((edge ? g.pzbz1 : g.pzbz) + ai * dzv) - (edge ? 1 : 0);
+      const int kz = min(max(lower ? k : k + 1, 0), nz);
+      const float az = s_vz[kz] * izv;
+      const float mz = a1 < az ? a1 : az;
+      const float w0 = mz - a0, w1 = a1 - mz;
+      const int k2 = k + dir;
+      const float *__restrict__ col = vol + (size_t)en.col * nz;
+      const float c0 = (k >= 0 && k < nz) ? col[k] : 0.f;
+      const float c1 = (k2 >= 0 && k2 < nz) ? col[k2] : 0.f;
+      acc = acc + (c0 * w0 + c1 * w1);
+      a0 = a1;
+    }
+  }
+  *out = acc * g.dconv[(size_t)h * g.nv + v];
+}
+
+// ---------------------------------------------------------------------------
+// backprojection
+// ---------------------------------------------------------------------------
+
+constexpr int kBpWaves = 4;
+constexpr int kBpCap = 256;  // LDS entries per wave per batch
+
+// Detector columns covered by the four corner lines of one voxel column
+// (c2D.cpp:837-901).
+__device__ inline void cone_col_hrange(const ConeArgs &g, const double *t,
+                                       double x0, double xn, double y0,
+                                       double yn, int &hmin, int &hmax) {
+  const double c = t[0], s = t[1], p1x = t[2], p1y = t[3];
+  const double dx0 = x0 - p1x, dy0 = y0 - p1y;
+  const double dxn = xn - p1x, dyn = yn - p1y;
+  const double cx0 = c * x0 - g.sx, cxn = c * xn - g.sx;
+  const double sy0 = s * y0, syn = s * yn;
+  double y00, y01, y10, y11;
+  if (fabs(s) > fabs(c)) {
+    const double il = t[7];
+    y00 = (c - dx0 / (sy0 + cx0)) * il;
+    y01 = (c - dx0 / (syn + cx0)) * il;
+    y10 = (c - dxn / (sy0 + cxn)) * il;
+    y11 = (c - dxn / (syn + cxn)) * il;
+  } else {
+    const double il = t[6];
+    y00 = (-s + dy0 / (cx0 + sy0)) * il;
+    y01 = (-s + dyn / (cx0 + syn)) * il;
+    y10 = (-s + dy0 / (cxn + sy0)) * il;
+    y11 = (-s + dyn / (cxn + syn)) * il;
+  }
+  const int h00 = (int)floor(y00 * g.ipix + g.hpix0);
+  const int h01 = (int)floor(y01 * g.ipix + g.hpix0);
+  const int h10 = (int)floor(y10 * g.ipix + g.hpix0);
+  const int h11 = (int)floor(y11 * g.ipix + g.hpix0);
+  hmin = max(min(min(h00, h01), min(h10, h11)), 0);
+  hmax = min(max(max(h00, h01), max(h10, h11)), g.nh - 1);
+}
+
+// Alpha interval of ray (a, h) inside the voxel column (c2D.cpp:904-936).
+__device__ inline bool cone_col_alpha(const ConeArgs &g, const double *t, int h,
+                                      double x0, double xn, double y0, double yn,
+                                      float &a0, float &a1) {
+  const double c = t[0], s = t[1], p1x = t[2], p1y = t[3];
+  const double p2x = t[4] - s * g.h_pix[h];
+  const double p2y = t[5] + c * g.h_pix[h];
+  const double ddx = p2x - p1x, ddy = p2y - p1y;
+  if (fabs(ddx) < kEps || fabs(ddy) < kEps) return false;
+  const double ix = 1.0 / ddx, iy = 1.0 / ddy;
+  const double ax0 = (x0 - p1x) * ix, ay0 = (y0 - p1y) * iy;
+  const double axn = (xn - p1x) * ix, ayn = (yn - p1y) * iy;
+  const double amin = fmax(fmax(fmin(ax0, axn), fmin(ay0, ayn)), 0.0);
+  const double amax = fmin(fmin(fmax(ax0, axn), fmax(ay0, ayn)), 1.0);
+  if (!(amin < amax - kEps)) return false;
+  a0 = (float)amin;
+  a1 = (float)amax;
+  return true;
+}
+
+__device__ inline int wave_excl_scan(int x, int lane, int &total) {
+  // inclusive scan over 64 lanes with shuffles
+  int incl = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  total = __shfl(incl, 63, 64);
+  return incl - x;
+}
+
+__global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
+    ConeArgs g, const float *__restrict__ dpix, float *__restrict__ vol,
+    int *__restrict__ err) {
+  extern __shared__ float smem[];
+  float *s_vz = smem;
+  const int nz = g.nz, nv = g.nv;
+  for (int q = threadIdx.x; q <= nz; q += 64 * kBpWaves) s_vz[q] = g.vz[q];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int colid = blockIdx.x * kBpWaves + wave;
+  if (colid >= g.nx * g.ny) return;
+  float *acc = smem + (nz + 1) + wave * (nz + 3 * kBpCap);
+  uint32_t *e_ray = (uint32_t *)(acc + nz);
+  float *e_a0 = acc + nz + kBpCap;
+  float *e_a1 = acc + nz + 2 * kBpCap;
+  const int i = colid / g.ny, j = colid - (colid / g.ny) * g.ny;
+  const double x0 = g.vx + (double)i * g.dx, xn = g.vx + (double)(i + 1) * g.dx;
+  const double y0 = g.vy + (double)j * g.dy, yn = g.vy + (double)(j + 1) * g.dy;
+  float *__restrict__ vcol = vol + (size_t)colid * nz;
+
+  // pass 1: rows that stay inside for every ray (c2D.cpp:657-676)
+  const float pzdeps = g.pzdv * (float)(1.0 + kEps);
+  int lo_max = 0, hi_min = nv;
+  for (int a = lane; a < g.na; a += 64) {
+    const double *t = g.atab + 8 * a;
+    int hmin, hmax;
+    cone_col_hrange(g, t, x0, xn, y0, yn, hmin, hmax);
+    for (int h = hmin; h <= hmax; h++) {
+      float a0, a1;
+      if (cone_col_alpha(g, t, h, x0, xn, y0, yn, a0, a1)) {
+        lo_max = max(lo_max, (int)ceilf(pzdeps + g.z1 / a0));
+        hi_min = min(hi_min, (int)floorf(g.pzdv + g.znm / a0));
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    lo_max = max(lo_max, __shfl_xor(lo_max, d, 64));
+    hi_min = min(hi_min, __shfl_xor(hi_min, d, 64));
+  }
+  const int min_v = lo_max, max_v = hi_min, mid = g.mid;
+  // every row in exactly one of the four row sets of c2D.cpp:687-749?
+  const bool one_set = (min_v <= mid) && (max_v >= mid);
+
+  for (int k = lane; k < nz; k += 64) acc[k] = vcol[k];
+
+  // pass 2: batches of angles -> LDS entry list -> row sweep
+  for (int base = 0; base < g.na; base += 64) {
+    const int a = base + lane;
+    int hmin = 0, hmax = -1;
+    const double *t = g.atab + 8 * (a < g.na ? a : 0);
+    if (a < g.na) cone_col_hrange(g, t, x0, xn, y0, yn, hmin, hmax);
+    int cnt = 0;
+    for (int h = hmin; h <= hmax; h++) {
+      float a0, a1;
+      cnt += cone_col_alpha(g, t, h, x0, xn, y0, yn, a0, a1) ? 1 : 0;
+    }
+    if (cnt > kBpCap) {  // one angle alone overflows a batch
+      atomicMax(err, 1);
+      cnt = 0;
+      hmax = hmin - 1;
+    }
+    bool pending = true;
+    while (__any(pending)) {
+      int total;
+      const int my = pending ? cnt : 0;
+      const int off = wave_excl_scan(my, lane, total);
+      const bool take = pending && (off + my <= kBpCap);
+      int used;
+      const int off2 = wave_excl_scan(take ? my : 0, lane, used);
+      if (take) {
+        int q = off2;
+        for (int h = hmin; h <= hmax; h++) {
+          float a0, a1;
+          if (cone_col_alpha(g, t, h, x0, xn, y0, yn, a0, a1)) {
+            e_ray[q] = (uint32_t)(a * g.nh + h);
+            e_a0[q] = a0;
+            e_a1[q] = a1;
+            ++q;
+          }
+        }
+        pending = false;
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      // sweep rows for every entry of the batch
+      const int npass = one_set ? 1 : 4;
+      for (int e = 0; e < used; ++e) {
+        const uint32_t ray = e_ray[e];
+        const float a0 = e_a0[e], a1 = e_a1[e];
+        const float ai = a0 * g.inv_dz;
+        const float *__restrict__ prow = dpix + (size_t)ray * nv;
+        for (int vb = 0; vb < nv; vb += 64) {
+          const int v = vb + lane;
+          if (v >= nv) break;
+          const float p = prow[v];
+          const float dzv = g.dz[v], izv = g.iz[v];
+          for (int pass = 0; pass < npass; ++pass) {
+            // row set: 0 lower edge, 1 lower main, 2 upper main, 3 upper edge
+            int set;
+            if (one_set) {
+              set = v < min_v ? 0 : (v < mid ? 1 : (v < max_v ? 2 : 3));
+            } else {
+              set = pass;
+              const bool member = (set == 0) ? (v < min_v)
+                                : (set == 1) ? (v >= min_v && v < mid)
+                                : (set == 2) ? (v >= mid && v < max_v)
+                                             : (v >= max_v);
+              if (!member) continue;
+            }
+            const bool lower = set <= 1;
+            const int k = (set == 0) ? (int)(g.pzbz1 + ai * dzv) - 1
+                                     : (int)(g.pzbz + ai * dzv);
+            const int kz = min(max(lower ? k : k + 1, 0), nz);
+            const float az = s_vz[kz] * izv;
+            const float mz = a1 < az ? a1 : az;
+            const int k2 = lower ? k - 1 : k + 1;
+            if (k >= 0 && k < nz) atomicAdd(&acc[k], p * (mz - a0));
+            if (k2 >= 0 && k2 < nz) atomicAdd(&acc[k2], p * (a1 - mz));
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  for (int k = lane; k < nz; k += 64) vcol[k] = acc[k];
+}
+
+// dpix = pix * d_conv (c2D.cpp:1105-1114); also clears the BP overflow flag
+// (a kernel store, so the sequence stays graph-capturable).
+__global__ void cone_scale_kernel(const float *__restrict__ pix,
+                                  const float *__restrict__ dconv,
+                                  float *__restrict__ dpix, size_t total,
+                                  int nhv, int *__restrict__ err) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *err = 0;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride)
+    dpix[q] = pix[q] * dconv[q % (size_t)nhv];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+void cone_build_plan(ConePlan &P, hipStream_t st) {
+  const Geometry &G = P.g;
+  const int na = G.na, nh = G.nh, nv = G.nv, nx = G.nx, ny = G.ny, nz = G.nz;
+  const double sx = G.source[0], sy = G.source[1], sz = G.source[2];
+  ConeArgs A{};
+  A.na = na; A.nh = nh; A.nv = nv; A.nx = nx; A.ny = ny; A.nz = nz;
+  // tables (c2D.cpp:480-516, 993-1010)
+  A.mid = nv;
+  for (int v = 0; v < nv; v++)
+    if (G.v_pix[v] > sz) { A.mid = v; break; }
+  A.inv_dz = (float)(1.0 / G.vsize[2]);
+  A.pzbz = (float)((sz - G.origin[2]) / G.vsize[2]);
+  A.pzbz1 = A.pzbz + 1.0f;
+  const double vstep = G.v_pix[1] - G.v_pix[0];
+  A.pzdv = (float)((sz - G.v_pix[0]) / vstep);
+  A.z1 = (float)((G.origin[2] + G.vsize[2] - sz) / vstep);
+  A.znm = (float)((G.origin[2] + (double)(nz - 1) * G.vsize[2] - sz) / vstep);
+  A.vx = G.origin[0]; A.vy = G.origin[1];
+  A.dx = G.vsize[0]; A.dy = G.vsize[1];
+  A.sx = sx; A.sy = sy; A.det_x = G.det_x;
+  A.L = G.det_x - sx;
+  const double step = G.h_pix[1] - G.h_pix[0];
+  A.ipix = 1.0 / step;
+  A.hpix0 = (sy - G.h_pix[0]) / step;
+
+  std::vector<float> dz(nv), iz(nv), vz(nz + 1), dconv((size_t)nh * nv);
+  for (int v = 0; v < nv; v++) {
+    dz[v] = (float)(G.v_pix[v] - sz);
+    iz[v] = (float)(1.0 / (double)dz[v]);
+  }
+  for (int k = 0; k <= nz; k++) vz[k] = (float)(G.origin[2] + (double)k * G.vsize[2] - sz);
+  const double x2 = (G.det_x - sx) * (G.det_x - sx);
+  for (int h = 0; h < nh; h++) {
+    const double xy2 = x2 + (G.h_pix[h] - sy) * (G.h_pix[h] - sy);
+    for (int v = 0; v < nv; v++)
+      dconv[(size_t)h * nv + v] =
+          (float)std::sqrt(xy2 + (G.v_pix[v] - sz) * (G.v_pix[v] - sz));
+  }
+  // per-angle tables and whole-volume footprint (c2D.cpp:549-600, 978-991)
+  std::vector<double> atab((size_t)na * 8);
+  std::vector<int> hr((size_t)na * 2);
+  const double vx = G.origin[0], wx = G.origin[0] + (double)nx * G.vsize[0];
+  const double vy = G.origin[1], wy = G.origin[1] + (double)ny * G.vsize[1];
+  for (int a = 0; a < na; a++) {
+    const double c = std::cos(G.angles[a]), s = std::sin(G.angles[a]);
+    double *t = &atab[(size_t)a * 8];
+    t[0] = c; t[1] = s;
+    t[2] = c * sx - s * sy;
+    t[3] = s * sx + c * sy;
+    t[4] = c * G.det_x;
+    t[5] = s * G.det_x;
+    t[6] = A.L / c;
+    t[7] = A.L / s;
+    const double p1x = t[2], p1y = t[3];
+    const double dx0 = vx - p1x, dy0 = vy - p1y, dxn = wx - p1x, dyn = wy - p1y;
+    const double cx0 = c * vx - sx, cxn = c * wx - sx, sy0 = s * vy, syn = s * wy;
+    double y00, y01, y10, y11;
+    if (std::fabs(s) > std::fabs(c)) {
+      const double il = A.L / s;
+      y00 = (c - dx0 / (sy0 + cx0)) * il;
+      y01 = (c - dx0 / (syn + cx0)) * il;
+      y10 = (c - dxn / (sy0 + cxn)) * il;
+      y11 = (c - dxn / (syn + cxn)) * il;
+    } else {
+      const double il = A.L / c;
+      y00 = (-s + dy0 / (cx0 + sy0)) * il;
+      y01 = (-s + dyn / (cx0 + syn)) * il;
+      y10 = (-s + dy0 / (cxn + sy0)) * il;
+      y11 = (-s + dyn / (cxn + syn)) * il;
+    }
+    const int h00 = (int)std::floor(y00 * A.ipix + A.hpix0);
+    const int h01 = (int)std::floor(y01 * A.ipix + A.hpix0);
+    const int h10 = (int)std::floor(y10 * A.ipix + A.hpix0);
+    const int h11 = (int)std::floor(y11 * A.ipix + A.hpix0);
+    hr[2 * a] = std::max(std::min(std::min(h00, h01), std::min(h10, h11)), 0);
+    hr[2 * a + 1] = std::min(std::max(std::max(h00, h01), std::max(h10, h11)), nh - 1);
+  }
+  P.atab.upload(atab, st);
+  P.hpix.upload(G.h_pix, st);
+  P.hrange.upload(hr, st);
+  P.dz.upload(dz, st);
+  P.iz.upload(iz, st);
+  P.vz.upload(vz, st);
+  P.dconv.upload(dconv, st);
+  A.atab = P.atab.as<double>();
+  A.h_pix = P.hpix.as<double>();
+  A.hrange = P.hrange.as<int>();
+  A.dz = P.dz.as<float>();
+  A.iz = P.iz.as<float>();
+  A.vz = P.vz.as<float>();
+  A.dconv = P.dconv.as<float>();
+
+  // walk: count, scan, fill
+  const size_t nrays = (size_t)na * nh;
+  DeviceBuffer counts;
+  counts.alloc(nrays * sizeof(uint32_t));
+  P.offs.alloc(nrays * sizeof(uint64_t));
+  P.rays.alloc(nrays * sizeof(ConeRay));
+  const int blocks = (int)((nrays + 255) / 256);
+  hipLaunchKernelGGL(cone_walk_kernel<false>, dim3(blocks), dim3(256), 0, st, A,
+                     nullptr, counts.as<uint32_t>(), nullptr, nullptr);
+  CCPI_HIP(hipGetLastError());
+  std::vector<uint32_t> hc(nrays);
+  CCPI_HIP(hipMemcpyAsync(hc.data(), counts.p, nrays * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, st));
+  CCPI_HIP(hipStreamSynchronize(st));
+  std::vector<uint64_t> ho(nrays);
+  uint64_t tot = 0;
+  for (size_t q = 0; q < nrays; q++) {
+    ho[q] = tot;
+    tot += hc[q];
+  }
+  P.n_entries = tot;
+  P.offs.upload(ho, st);
+  P.ents.alloc(std::max<uint64_t>(tot, 1) * sizeof(ConeEnt));
+  hipLaunchKernelGGL(cone_walk_kernel<true>, dim3(blocks), dim3(256), 0, st, A,
+                     P.offs.as<uint64_t>(), nullptr, P.rays.as<ConeRay>(),
+                     P.ents.as<ConeEnt>());
+  CCPI_HIP(hipGetLastError());
+  CCPI_HIP(hipStreamSynchronize(st));
+  P.args = A;
+}
+
+void cone_fp(const ConePlan &P, const float *vol, float *sino, hipStream_t st) {
+  const ConeArgs &A = P.args;
+  dim3 grid((unsigned)(A.na * A.nh), (unsigned)((A.nv + kFpRows - 1) / kFpRows));
+  const size_t lds = sizeof(float) * (size_t)(A.nz + 1);
+  hipLaunchKernelGGL(cone_fp_kernel, grid, dim3(kFpRows), lds, st, A,
+                     P.rays.as<ConeRay>(), P.ents.as<ConeEnt>(), vol, sino);
+  CCPI_HIP(hipGetLastError());
+}
+
+size_t cone_bp_lds_bytes(int nz) {
+  return sizeof(float) * ((size_t)(nz + 1) + (size_t)kBpWaves * (nz + 3 * kBpCap));
+}
+
+void cone_scale(const ConePlan &P, const float *pix, float *dpix, int *err,
+                hipStream_t st) {
+  const ConeArgs &A = P.args;
+  const size_t total = (size_t)A.na * A.nh * A.nv;
+  const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(cone_scale_kernel, dim3(blocks), dim3(256), 0, st, pix,
+                     A.dconv, dpix, total, A.nh * A.nv, err);
+  CCPI_HIP(hipGetLastError());
+}
+
+void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, int *err,
+                    hipStream_t st) {
+  const ConeArgs &A = P.args;
+  const size_t lds = cone_bp_lds_bytes(A.nz);
+  CCPI_REQUIRE(lds <= 160 * 1024, CCPI_E_LIMIT, "cone BP: nz too large for LDS");
+  const int cols = A.nx * A.ny;
+  const int blocks = (cols + kBpWaves - 1) / kBpWaves;
+  hipLaunchKernelGGL(cone_bp_kernel, dim3(blocks), dim3(64 * kBpWaves), lds, st,
+                     A, dpix, vol, err);
+  CCPI_HIP(hipGetLastError());
+}
+
+}  // namespace ccpi

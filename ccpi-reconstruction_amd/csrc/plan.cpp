@@ -1,0 +1,226 @@
+// Geometry validation and the per-device plan cache.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <list>
+#include <map>
+#include <mutex>
+
+namespace ccpi {
+
+static std::vector<double> copy_arr(const double *p, int n) {
+  return std::vector<double>(p, p + n);
+}
+
+Geometry Geometry::from(const ccpi_geometry *g) {
+  CCPI_REQUIRE(g, CCPI_E_ARG, "null geometry");
+  CCPI_REQUIRE(g->angles && g->h_pixels && g->v_pixels, CCPI_E_ARG,
+               "null geometry array");
+  CCPI_REQUIRE(g->n_angles > 0 && g->n_h > 0 && g->n_v > 0, CCPI_E_ARG,
+               "non-positive detector size");
+  Geometry G;
+  G.kind = g->kind;
+  G.na = g->n_angles; G.nh = g->n_h; G.nv = g->n_v;
+  G.nx = g->nx; G.ny = g->ny; G.nz = g->nz;
+  G.angles = copy_arr(g->angles, g->n_angles);
+  G.h_pix = copy_arr(g->h_pixels, g->n_h);
+  G.v_pix = copy_arr(g->v_pixels, g->n_v);
+  for (int q = 0; q < 3; q++) {
+    G.source[q] = g->source[q];
+    G.origin[q] = g->origin[q];
+    G.vsize[q] = g->voxel_size[q];
+  }
+  G.det_x = g->detector_x;
+  G.validate();
+  return G;
+}
+
+Geometry Geometry::parallel(const double *h, const double *v, const double *ang,
+                            int na, int nh, int nv, const double o[3],
+                            const double s[3], int nx, int ny, int nz) {
+  CCPI_REQUIRE(h && v && ang && o && s, CCPI_E_ARG, "null geometry array");
+  CCPI_REQUIRE(na > 0 && nh > 0 && nv > 0, CCPI_E_ARG, "non-positive detector size");
+  Geometry G;
+  G.kind = CCPI_PARALLEL;
+  G.na = na; G.nh = nh; G.nv = nv; G.nx = nx; G.ny = ny; G.nz = nz;
+  G.angles = copy_arr(ang, na);
+  G.h_pix = copy_arr(h, nh);
+  G.v_pix = copy_arr(v, nv);
+  for (int q = 0; q < 3; q++) { G.origin[q] = o[q]; G.vsize[q] = s[q]; }
+  G.validate();
+  return G;
+}
+
+Geometry Geometry::cone(double sx, double sy, double sz, double dx,
+                        const double *h, const double *v, const double *ang,
+                        int na, int nh, int nv, const double o[3],
+                        const double s[3], int nx, int ny, int nz) {
+  Geometry G = parallel(h, v, ang, na, nh, nv, o, s, nx, ny, nz);
+  G.kind = CCPI_CONE;
+  G.source[0] = sx; G.source[1] = sy; G.source[2] = sz;
+  G.det_x = dx;
+  G.validate();
+  return G;
+}
+
+void Geometry::validate() const {
+  CCPI_REQUIRE(kind == CCPI_PARALLEL || kind == CCPI_CONE, CCPI_E_ARG, "bad kind");
+  CCPI_REQUIRE(nx > 0 && ny > 0 && nz > 0, CCPI_E_ARG, "non-positive volume size");
+  CCPI_REQUIRE((long long)nx * ny < (1LL << 31), CCPI_E_LIMIT, "nx*ny too large");
+  for (int q = 0; q < 3; q++)
+    CCPI_REQUIRE(vsize[q] > 0 && std::isfinite(vsize[q]) && std::isfinite(origin[q]),
+                 CCPI_E_GEOMETRY, "voxel size must be positive and finite");
+  // the detector index map uses the first pixel step (p2D.cpp:493-494,
+  // c2D.cpp:518-520)
+  CCPI_REQUIRE(nh >= 2, CCPI_E_GEOMETRY, "need at least two detector columns");
+  for (int h = 1; h < nh; h++)
+    CCPI_REQUIRE(h_pix[h] > h_pix[h - 1], CCPI_E_GEOMETRY,
+                 "h pixel positions must increase");
+  for (int v = 1; v < nv; v++)
+    CCPI_REQUIRE(v_pix[v] > v_pix[v - 1], CCPI_E_GEOMETRY,
+                 "v pixel positions must increase");
+  for (double a : angles) CCPI_REQUIRE(std::isfinite(a), CCPI_E_GEOMETRY, "bad angle");
+  if (kind == CCPI_CONE) {
+    // b2D divides by nz - 1 (c2D.cpp:808) and uses v_pix[1] - v_pix[0]
+    CCPI_REQUIRE(nz >= 2, CCPI_E_GEOMETRY, "cone beam needs nz >= 2");
+    CCPI_REQUIRE(nv >= 2, CCPI_E_GEOMETRY, "cone beam needs two detector rows");
+    // a row at the source height has no defined slope (c2D.cpp:487-492)
+    for (int v = 0; v < nv; v++)
+      CCPI_REQUIRE(std::fabs(v_pix[v] - source[2]) >= kEps, CCPI_E_GEOMETRY,
+                   "detector row at the source height");
+    // the walk starts at the source, which must lie outside the volume's
+    // xy footprint at every angle: outside its circumscribed circle
+    double R = 0.0;   // farthest volume corner from the rotation axis
+    for (int cx = 0; cx < 2; cx++)
+      for (int cy = 0; cy < 2; cy++)
+        R = std::max(R, std::hypot(origin[0] + cx * nx * vsize[0],
+                                   origin[1] + cy * ny * vsize[1]));
+    CCPI_REQUIRE(std::hypot(source[0], source[1]) > R, CCPI_E_GEOMETRY,
+                 "source inside the volume's rotation circle");
+    CCPI_REQUIRE(det_x > source[0], CCPI_E_GEOMETRY,
+                 "detector must be on the far side of the source");
+  }
+}
+
+std::string Geometry::key() const {
+  std::string k;
+  auto put = [&](const void *p, size_t n) { k.append((const char *)p, n); };
+  int hdr[7] = {kind, na, nh, nv, nx, ny, nz};
+  put(hdr, sizeof hdr);
+  put(angles.data(), angles.size() * 8);
+  put(h_pix.data(), h_pix.size() * 8);
+  put(v_pix.data(), v_pix.size() * 8);
+  put(source, sizeof source);
+  put(&det_x, sizeof det_x);
+  put(origin, sizeof origin);
+  put(vsize, sizeof vsize);
+  return k;
+}
+
+// ---- cache ---------------------------------------------------------------
+
+namespace {
+struct Entry {
+  std::shared_ptr<ConePlan> cone;
+  std::shared_ptr<ParPlan> par;
+  size_t bytes() const { return cone ? cone->bytes() : par->bytes(); }
+};
+struct Cache {
+  std::mutex mu;
+  std::list<std::pair<std::string, Entry>> lru;  // front = most recent
+};
+Cache &cache() {
+  static Cache c;
+  return c;
+}
+constexpr size_t kMaxPlans = 8;
+
+std::string dev_key(const Geometry &g) {
+  int dev = 0;
+  CCPI_HIP(hipGetDevice(&dev));
+  return std::to_string(dev) + ":" + g.key();
+}
+
+Entry *lookup(Cache &c, const std::string &k) {
+  for (auto it = c.lru.begin(); it != c.lru.end(); ++it)
+    if (it->first == k) {
+      c.lru.splice(c.lru.begin(), c.lru, it);
+      return &c.lru.front().second;
+    }
+  return nullptr;
+}
+
+void insert(Cache &c, const std::string &k, Entry e) {
+  c.lru.emplace_front(k, std::move(e));
+  while (c.lru.size() > kMaxPlans) c.lru.pop_back();
+}
+}  // namespace
+
+std::shared_ptr<ConePlan> cone_plan(const Geometry &g, hipStream_t st) {
+  Cache &c = cache();
+  const std::string k = dev_key(g);
+  std::lock_guard<std::mutex> lock(c.mu);
+  if (Entry *e = lookup(c, k)) return e->cone;
+  auto P = std::make_shared<ConePlan>();
+  P->g = g;
+  cone_build_plan(*P, st);
+  insert(c, k, Entry{P, nullptr});
+  return P;
+}
+
+std::shared_ptr<ParPlan> par_plan(const Geometry &g, hipStream_t st) {
+  Cache &c = cache();
+  const std::string k = dev_key(g);
+  std::lock_guard<std::mutex> lock(c.mu);
+  if (Entry *e = lookup(c, k)) return e->par;
+  auto P = std::make_shared<ParPlan>();
+  P->g = g;
+  par_build_plan(*P, st);
+  insert(c, k, Entry{nullptr, P});
+  return P;
+}
+
+void plan_cache_clear() {
+  Cache &c = cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  c.lru.clear();
+}
+
+long long plan_cache_bytes() {
+  Cache &c = cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  long long b = 0;
+  for (auto &e : c.lru) b += (long long)e.second.bytes();
+  return b;
+}
+
+// ---- projector pair -------------------------------------------------------
+
+Projector::Projector(const Geometry &g, hipStream_t st) {
+  if (g.kind == CCPI_CONE)
+    cone = cone_plan(g, st);
+  else
+    par = par_plan(g, st);
+}
+
+void Projector::fp(const float *vol, float *sino, hipStream_t st) const {
+  if (cone)
+    cone_fp(*cone, vol, sino, st);
+  else
+    par_fp(*par, vol, sino, st);
+}
+
+void Projector::bp(const float *sino, float *vol, hipStream_t st) const {
+  if (cone) {
+    ConePlan &P = *cone;
+    P.dpix.alloc(P.g.n_pix() * sizeof(float));
+    P.err.alloc(sizeof(int));
+    cone_scale(P, sino, P.dpix.as<float>(), P.err.as<int>(), st);
+    cone_bp_scaled(P, P.dpix.as<float>(), vol, P.err.as<int>(), st);
+  } else {
+    par_bp(*par, sino, vol, st);
+  }
+}
+
+}  // namespace ccpi

@@ -1,0 +1,129 @@
+// Per-geometry device plans and their cache.
+//
+// A plan holds everything about one projector instance that does not
+// depend on the data: per-angle / per-row / per-slice tables and, for the
+// cone beam, the xy walk of every ray.  Plans are built once and reused by
+// every projector call on the same geometry (the reference recomputes all
+// of it inside every f2D / b2D call, c2D.cpp:480-516, p2D.cpp:477-495).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace ccpi {
+
+// Owned copy of a ccpi_geometry.
+struct Geometry {
+  int kind = CCPI_PARALLEL;
+  int na = 0, nh = 0, nv = 0, nx = 0, ny = 0, nz = 0;
+  std::vector<double> angles, h_pix, v_pix;
+  double source[3] = {0, 0, 0};
+  double det_x = 0;
+  double origin[3] = {0, 0, 0};
+  double vsize[3] = {0, 0, 0};
+
+  static Geometry from(const ccpi_geometry *g);
+  static Geometry parallel(const double *h, const double *v, const double *ang,
+                           int na, int nh, int nv, const double o[3],
+                           const double s[3], int nx, int ny, int nz);
+  static Geometry cone(double sx, double sy, double sz, double dx,
+                       const double *h, const double *v, const double *ang,
+                       int na, int nh, int nv, const double o[3],
+                       const double s[3], int nx, int ny, int nz);
+  // Throws Error(CCPI_E_ARG / CCPI_E_GEOMETRY) for inputs the model cannot
+  // represent.
+  void validate() const;
+  std::string key() const;
+  size_t n_vox() const { return (size_t)nx * ny * nz; }
+  size_t n_pix() const { return (size_t)na * nh * nv; }
+};
+
+// Owning device allocation.
+struct DeviceBuffer {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer &) = delete;
+  DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+  ~DeviceBuffer() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t n) {
+    if (n <= bytes && p) return;
+    reset();
+    CCPI_HIP(hipMalloc(&p, n ? n : 4));
+    bytes = n;
+  }
+  template <class T>
+  void upload(const std::vector<T> &v, hipStream_t st) {
+    alloc(v.size() * sizeof(T));
+    if (!v.empty())
+      CCPI_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T),
+                              hipMemcpyHostToDevice, st));
+  }
+  template <class T>
+  T *as() const { return static_cast<T *>(p); }
+};
+
+struct ConePlan {
+  Geometry g;
+  ConeArgs args{};
+  DeviceBuffer atab, hpix, hrange, dz, iz, vz, dconv, offs, rays, ents;
+  DeviceBuffer dpix;    // scaled sinogram scratch of the BP
+  DeviceBuffer err;     // BP overflow flag
+  uint64_t n_entries = 0;
+  size_t bytes() const {
+    return atab.bytes + hpix.bytes + hrange.bytes + dz.bytes + iz.bytes +
+           vz.bytes + dconv.bytes + offs.bytes + rays.bytes + ents.bytes +
+           dpix.bytes + err.bytes;
+  }
+};
+
+struct ParPlan {
+  Geometry g;
+  ParArgs args{};
+  DeviceBuffer atab, hpix, map, rlo, rhi;
+  size_t bytes() const {
+    return atab.bytes + hpix.bytes + map.bytes + rlo.bytes + rhi.bytes;
+  }
+};
+
+std::shared_ptr<ConePlan> cone_plan(const Geometry &g, hipStream_t st);
+std::shared_ptr<ParPlan> par_plan(const Geometry &g, hipStream_t st);
+void plan_cache_clear();
+long long plan_cache_bytes();
+
+// launchers (cone.hip, parallel.hip)
+void cone_build_plan(ConePlan &P, hipStream_t st);
+void cone_fp(const ConePlan &P, const float *vol, float *sino, hipStream_t st);
+void cone_scale(const ConePlan &P, const float *pix, float *dpix, int *err,
+                hipStream_t st);
+void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, int *err,
+                    hipStream_t st);
+size_t cone_bp_lds_bytes(int nz);
+
+void par_build_plan(ParPlan &P, hipStream_t st);
+void par_fp(const ParPlan &P, const float *vol, float *sino, hipStream_t st);
+void par_bp(const ParPlan &P, const float *sino, float *vol, hipStream_t st);
+
+// Projector pair on one geometry with device-resident plan; BP leaves the
+// sinogram untouched.
+struct Projector {
+  std::shared_ptr<ConePlan> cone;
+  std::shared_ptr<ParPlan> par;
+  explicit Projector(const Geometry &g, hipStream_t st);
+  void fp(const float *vol, float *sino, hipStream_t st) const;
+  void bp(const float *sino, float *vol, hipStream_t st) const;
+  const Geometry &geometry() const { return cone ? cone->g : par->g; }
+};
+
+}  // namespace ccpi

@@ -1,0 +1,352 @@
+// CGLS / SIRT / MLEM iteration bodies on device-resident vectors.
+//
+// Bodies follow Core/src/Algorithms/cgls.cpp:26-157 (cgls_3d),
+// sirt.cpp:15-80 and mlem.cpp:15-80 with the vector operations of
+// Core/src/blas.hpp:11-466.  Differences by design:
+//   * reductions (||Ad||^2, ||A^T r||^2) are deterministic two-pass
+//     reductions: per-workgroup partials, then one fixed-order pass.  By
+//     default partials and the CGLS scalars are FP64 (the steps whose
+//     stability needs it); CCPI_REDUCE_FP32 keeps float partials and a float
+//     alpha like the reference (cgls.cpp:118-119);
+//   * alpha / beta never leave the device, so an iteration issues no host
+//     synchronisation and is graph-capturable;
+//   * element-wise steps are fused (b - r then * R; x + t * C then clamp;
+//     ratio then clamp; x * N * t then clamp).
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "common.hpp"
+#include "plan.hpp"
+#include "solvers.hpp"
+
+namespace ccpi {
+
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+
+__global__ void fill_kernel(float *__restrict__ p, size_t n, float val) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+    p[q] = val;
+}
+
+template <class Acc>
+__device__ Acc block_sum(Acc v) {
+  __shared__ Acc sh[kRedThreads / 64];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  Acc t = 0;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < kRedThreads / 64; q++) t += sh[q];
+  return t;
+}
+
+// Per-workgroup partial sums of x^2 (fixed grid: deterministic).
+template <class Acc>
+__global__ __launch_bounds__(kRedThreads) void sumsq_kernel(const float *__restrict__ x,
+                                                            size_t n, Acc *__restrict__ part) {
+  Acc s = 0;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+    const Acc v = (Acc)x[q];
+    s += v * v;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+template <class Acc>
+__device__ Acc sum_partials(const Acc *part) {
+  // fixed-order pairwise tree over kRedBlocks partials in one block
+  __shared__ Acc sh[kRedThreads];
+  Acc s = 0;
+  for (int q = threadIdx.x; q < kRedBlocks; q += kRedThreads) s += part[q];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// gamma = ||d||^2 at start (cgls.cpp:48-49)
+template <class Acc>
+__global__ __launch_bounds__(kRedThreads) void cgls_init_kernel(const Acc *part,
+                                                                CglsScalars *sc) {
+  const Acc s = sum_partials(part);
+  if (threadIdx.x == 0) {
+    sc->gamma = (double)s;
+    sc->gamma_f = (float)s;
+  }
+}
+
+// alpha = gamma / ||Ad||^2 (cgls.cpp:118-119)
+template <class Acc>
+__global__ __launch_bounds__(kRedThreads) void cgls_alpha_kernel(const Acc *part,
+                                                                 CglsScalars *sc, int fp32) {
+  const Acc s = sum_partials(part);
+  if (threadIdx.x == 0) {
+    if (fp32) {
+      const float a = sc->gamma_f / (float)s;
+      sc->alpha_f = a;
+    } else {
+      sc->alpha_f = (float)(sc->gamma / (double)s);
+    }
+  }
+}
+
+// beta = ||s||^2 / gamma, gamma <- ||s||^2, record ||A^T r|| (cgls.cpp:153-156)
+template <class Acc>
+__global__ __launch_bounds__(kRedThreads) void cgls_beta_kernel(const Acc *part,
+                                                                CglsScalars *sc,
+                                                                double *norms, int it,
+                                                                int fp32) {
+  const Acc s = sum_partials(part);
+  if (threadIdx.x == 0) {
+    if (fp32) {
+      const float nn = (float)s;
+      const double beta = (double)nn / (double)sc->gamma_f;
+      sc->beta_f = (float)beta;
+      sc->gamma_f = nn;
+      norms[it] = (double)sqrtf(nn);
+    } else {
+      const double nn = (double)s;
+      sc->beta_f = (float)(nn / sc->gamma);
+      sc->gamma = nn;
+      norms[it] = sqrt(nn);
+    }
+  }
+}
+
+// x += alpha d (voxels) and r -= alpha Ad (pixels) (cgls.cpp:120-121)
+__global__ void cgls_update_xr_kernel(float *__restrict__ x, const float *__restrict__ d,
+                                      size_t nvox, float *__restrict__ r,
+                                      const float *__restrict__ Ad, size_t npix,
+                                      const CglsScalars *__restrict__ sc) {
+  const float a = sc->alpha_f;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (size_t q = t0; q < nvox; q += stride) x[q] = x[q] + a * d[q];
+  for (size_t q = t0; q < npix; q += stride) r[q] = r[q] + (-a) * Ad[q];
+}
+
+// d = s + beta d (scal_xby, blas.hpp:362-376)
+__global__ void cgls_update_d_kernel(float *__restrict__ d, const float *__restrict__ s,
+                                     size_t n, const CglsScalars *__restrict__ sc) {
+  const float b = sc->beta_f;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+    d[q] = b * d[q] + s[q];
+}
+
+// x = 1/x where x > m, else m (invert_min_x, blas.hpp:100-116)
+__global__ void invert_min_kernel(float *__restrict__ x, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+    x[q] = x[q] > 0.f ? (float)(1.0 / (double)x[q]) : 0.f;
+}
+
+// x = 1/x where x != 0 (invert_x, blas.hpp:86-98)
+__global__ void invert_nz_kernel(float *__restrict__ x, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+    if (x[q] != 0.f) x[q] = (float)(1.0 / (double)x[q]);
+}
+
+// r = (b - r) * R (sirt.cpp:60-62)
+__global__ void sirt_residual_kernel(float *__restrict__ r, const float *__restrict__ b,
+                                     const float *__restrict__ R, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+    const float t = -1.0f * r[q] + b[q];
+    r[q] = t * R[q];
+  }
+}
+
+// x = max(x + t * C, 0) (sirt.cpp:70-74)
+__global__ void sirt_update_kernel(float *__restrict__ x, const float *__restrict__ t,
+                                   const float *__restrict__ C, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+    const float v = x[q] + 1.0f * (t[q] * C[q]);
+    x[q] = v < 0.f ? 0.f : v;
+  }
+}
+
+// ratio = max(b / r or 0, 0) (mlem.cpp:54-65)
+__global__ void mlem_ratio_kernel(float *__restrict__ r, const float *__restrict__ b, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+    const float e = r[q];
+    const float v = e != 0.f ? b[q] / e : 0.f;
+    r[q] = v < 0.f ? 0.f : v;
+  }
+}
+
+// x = max(x * (N * t), 0) (mlem.cpp:72-74)
+__global__ void mlem_update_kernel(float *__restrict__ x, const float *__restrict__ N,
+                                   const float *__restrict__ t, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+    const float v = x[q] * (N[q] * t[q]);
+    x[q] = v < 0.f ? 0.f : v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+
+static int grid_for(size_t n) {
+  const size_t b = (n + 255) / 256;
+  return (int)std::min<size_t>(std::max<size_t>(b, 1), 8192);
+}
+
+void dev_fill(float *p, size_t n, float v, hipStream_t st) {
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, st, p, n, v);
+  CCPI_HIP(hipGetLastError());
+}
+
+Workspace::~Workspace() = default;
+
+void Workspace::ensure(size_t nvox, size_t npix) {
+  v1.alloc(nvox * 4);
+  v2.alloc(nvox * 4);
+  v3.alloc(nvox * 4);
+  p1.alloc(npix * 4);
+  p2.alloc(npix * 4);
+  part.alloc(kRedBlocks * sizeof(double));
+  scal.alloc(sizeof(CglsScalars));
+}
+
+template <class Acc>
+static void sumsq(const float *x, size_t n, void *part, hipStream_t st) {
+  hipLaunchKernelGGL(sumsq_kernel<Acc>, dim3(kRedBlocks), dim3(kRedThreads), 0, st, x, n,
+                     (Acc *)part);
+  CCPI_HIP(hipGetLastError());
+}
+
+void CglsRun::start(hipStream_t st) {
+  const size_t nvox = P.geometry().n_vox(), npix = P.geometry().n_pix();
+  W.ensure(nvox, npix);
+  norms_dev.alloc((size_t)std::max(iters, 1) * sizeof(double));
+  float *d = W.v1.as<float>();
+  dev_fill(d, nvox, 0.f, st);
+  P.bp(b, d, st);
+  if (fp32) {
+    sumsq<float>(d, nvox, W.part.p, st);
+    hipLaunchKernelGGL(cgls_init_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
+                       W.part.as<float>(), W.scal.as<CglsScalars>());
+  } else {
+    sumsq<double>(d, nvox, W.part.p, st);
+    hipLaunchKernelGGL(cgls_init_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
+                       W.part.as<double>(), W.scal.as<CglsScalars>());
+  }
+  CCPI_HIP(hipGetLastError());
+}
+
+void CglsRun::iterate(int it, hipStream_t st) {
+  const size_t nvox = P.geometry().n_vox(), npix = P.geometry().n_pix();
+  float *d = W.v1.as<float>(), *s = W.v2.as<float>(), *Ad = W.p1.as<float>();
+  CglsScalars *sc = W.scal.as<CglsScalars>();
+  dev_fill(Ad, npix, 0.f, st);
+  P.fp(d, Ad, st);
+  if (fp32) {
+    sumsq<float>(Ad, npix, W.part.p, st);
+    hipLaunchKernelGGL(cgls_alpha_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
+                       W.part.as<float>(), sc, 1);
+  } else {
+    sumsq<double>(Ad, npix, W.part.p, st);
+    hipLaunchKernelGGL(cgls_alpha_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
+                       W.part.as<double>(), sc, 0);
+  }
+  hipLaunchKernelGGL(cgls_update_xr_kernel, dim3(grid_for(std::max(nvox, npix))), dim3(256),
+                     0, st, x, d, nvox, b, Ad, npix, sc);
+  dev_fill(s, nvox, 0.f, st);
+  P.bp(b, s, st);
+  if (fp32) {
+    sumsq<float>(s, nvox, W.part.p, st);
+    hipLaunchKernelGGL(cgls_beta_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
+                       W.part.as<float>(), sc, norms_dev.as<double>(), it, 1);
+  } else {
+    sumsq<double>(s, nvox, W.part.p, st);
+    hipLaunchKernelGGL(cgls_beta_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
+                       W.part.as<double>(), sc, norms_dev.as<double>(), it, 0);
+  }
+  hipLaunchKernelGGL(cgls_update_d_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, d, s, nvox,
+                     sc);
+  CCPI_HIP(hipGetLastError());
+}
+
+void CglsRun::finish(double *norms, hipStream_t st) {
+  if (norms && iters > 0) {
+    CCPI_HIP(hipMemcpyAsync(norms, norms_dev.p, (size_t)iters * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    CCPI_HIP(hipStreamSynchronize(st));
+  }
+}
+
+void run_cgls(const Projector &P, float *b, float *x, int iters, double *norms, bool fp32,
+              Workspace &W, hipStream_t st) {
+  CglsRun run{P, W, b, x, iters, fp32};
+  run.start(st);
+  for (int it = 0; it < iters; it++) run.iterate(it, st);
+  run.finish(norms, st);
+}
+
+void run_sirt(const Projector &P, const float *b, float *x, int iters, Workspace &W,
+              hipStream_t st) {
+  const size_t nvox = P.geometry().n_vox(), npix = P.geometry().n_pix();
+  W.ensure(nvox, npix);
+  float *C = W.v1.as<float>(), *t = W.v2.as<float>(), *ones = W.v3.as<float>();
+  float *R = W.p1.as<float>(), *r = W.p2.as<float>();
+  // R = 1 / (A 1), C = 1 / (A^T 1) (sirt.cpp:31-48)
+  dev_fill(ones, nvox, 1.f, st);
+  dev_fill(R, npix, 0.f, st);
+  P.fp(ones, R, st);
+  hipLaunchKernelGGL(invert_min_kernel, dim3(grid_for(npix)), dim3(256), 0, st, R, npix);
+  dev_fill(r, npix, 1.f, st);
+  dev_fill(C, nvox, 0.f, st);
+  P.bp(r, C, st);
+  hipLaunchKernelGGL(invert_min_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, C, nvox);
+  for (int it = 0; it < iters; it++) {
+    dev_fill(r, npix, 0.f, st);
+    P.fp(x, r, st);
+    hipLaunchKernelGGL(sirt_residual_kernel, dim3(grid_for(npix)), dim3(256), 0, st, r, b, R,
+                       npix);
+    dev_fill(t, nvox, 0.f, st);
+    P.bp(r, t, st);
+    hipLaunchKernelGGL(sirt_update_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, x, t, C,
+                       nvox);
+  }
+  CCPI_HIP(hipGetLastError());
+}
+
+void run_mlem(const Projector &P, const float *b, float *x, int iters, Workspace &W,
+              hipStream_t st) {
+  const size_t nvox = P.geometry().n_vox(), npix = P.geometry().n_pix();
+  W.ensure(nvox, npix);
+  float *N = W.v1.as<float>(), *t = W.v2.as<float>();
+  float *r = W.p1.as<float>();
+  // x = 1; N = 1 / (A^T 1) keeping zeros (mlem.cpp:30-41)
+  dev_fill(x, nvox, 1.f, st);
+  dev_fill(r, npix, 1.f, st);
+  dev_fill(N, nvox, 0.f, st);
+  P.bp(r, N, st);
+  hipLaunchKernelGGL(invert_nz_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, N, nvox);
+  for (int it = 0; it < iters; it++) {
+    dev_fill(r, npix, 0.f, st);
+    P.fp(x, r, st);
+    hipLaunchKernelGGL(mlem_ratio_kernel, dim3(grid_for(npix)), dim3(256), 0, st, r, b, npix);
+    dev_fill(t, nvox, 0.f, st);
+    P.bp(r, t, st);
+    hipLaunchKernelGGL(mlem_update_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, x, N, t,
+                       nvox);
+  }
+  CCPI_HIP(hipGetLastError());
+}
+
+}  // namespace ccpi

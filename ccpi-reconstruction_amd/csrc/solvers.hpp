@@ -1,0 +1,50 @@
+// Device-resident iteration bodies (see solvers.hip).
+#pragma once
+
+#include "plan.hpp"
+
+namespace ccpi {
+
+// Scalars of one CGLS run, kept in device memory.
+struct CglsScalars {
+  double gamma;      // ||A^T r||^2 (FP64 mode)
+  float gamma_f;     // same, float mode
+  float alpha_f;     // step applied to x and r
+  float beta_f;      // direction update factor
+  float pad;
+};
+
+// Work vectors reused across calls on one geometry.
+struct Workspace {
+  DeviceBuffer v1, v2, v3;   // volume-sized
+  DeviceBuffer p1, p2;       // sinogram-sized
+  DeviceBuffer part;         // reduction partials
+  DeviceBuffer scal;         // CglsScalars
+  ~Workspace();
+  void ensure(size_t nvox, size_t npix);
+};
+
+// One CGLS run split in launch-only steps (no host sync inside
+// start / iterate), so a caller may capture iterate() in a graph.
+struct CglsRun {
+  const Projector &P;
+  Workspace &W;
+  float *b;      // sinogram in, residual out
+  float *x;      // volume in/out
+  int iters;
+  bool fp32;
+  DeviceBuffer norms_dev;
+  void start(hipStream_t st);
+  void iterate(int it, hipStream_t st);
+  void finish(double *norms, hipStream_t st);
+};
+
+void dev_fill(float *p, size_t n, float v, hipStream_t st);
+void run_cgls(const Projector &P, float *b, float *x, int iters, double *norms,
+              bool fp32, Workspace &W, hipStream_t st);
+void run_sirt(const Projector &P, const float *b, float *x, int iters,
+              Workspace &W, hipStream_t st);
+void run_mlem(const Projector &P, const float *b, float *x, int iters,
+              Workspace &W, hipStream_t st);
+
+}  // namespace ccpi

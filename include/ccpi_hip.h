@@ -1,0 +1,142 @@
+/*
+ * ccpi_hip.h - C ABI of the MI355X projector + iterative-solver library
+ * (libccpi_hip.so).
+ *
+ * Conventions (same as the reference, Core/src/base_types.hpp:31-63):
+ *   volume    float [nx][ny][nz], z fastest      ("voxels")
+ *   sinogram  float [n_angles][n_h][n_v], v fastest ("pixels")
+ *   geometry  double, host memory: angles in radians, detector pixel
+ *             positions and volume origin / voxel size in mm.
+ * Every float array argument is DEVICE memory on the current HIP device.
+ * Calls are asynchronous on `stream` (a hipStream_t, NULL = default
+ * stream); host outputs (norm arrays) are valid when the call returns.
+ * Every entry point returns 0 on success or a negative CCPI_E_* code; no
+ * C++ exception crosses this boundary.  No entry point frees or keeps
+ * caller memory.
+ */
+#ifndef CCPI_HIP_H
+#define CCPI_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define CCPI_OK              0
+#define CCPI_E_ARG          -1  /* null pointer / non-positive size */
+#define CCPI_E_GEOMETRY     -2  /* geometry the model cannot represent */
+#define CCPI_E_HIP          -3  /* HIP runtime error */
+#define CCPI_E_NOMEM        -4  /* device allocation failed */
+#define CCPI_E_LIMIT        -5  /* size beyond a kernel limit */
+
+/* flags for the solver entry points */
+#define CCPI_REDUCE_FP32     1  /* alpha/beta and norms in float, as the
+                                   reference (cgls.cpp:118-121, blas.hpp:204-305);
+                                   default is FP64 reductions */
+
+#define CCPI_PARALLEL 0
+#define CCPI_CONE     1
+
+/* One projector instance (instrument state of CCPi::parallel_beam /
+   CCPi::cone_beam, Core/src/instruments.hpp:130-143, 194-199). */
+typedef struct ccpi_geometry {
+  int kind;                 /* CCPI_PARALLEL or CCPI_CONE */
+  int n_angles, n_h, n_v;
+  const double *angles;     /* [n_angles] radians, host */
+  const double *h_pixels;   /* [n_h] mm, host, strictly increasing */
+  const double *v_pixels;   /* [n_v] mm, host, strictly increasing */
+  double source[3];         /* cone: source at angle 0 (x, y, z) mm */
+  double detector_x;        /* cone: detector plane x at angle 0, mm */
+  double origin[3];         /* volume corner, mm */
+  double voxel_size[3];     /* mm */
+  int nx, ny, nz;
+} ccpi_geometry;
+
+/* ---- projectors: the static f2D / b2D contract ------------------------ */
+
+/* y += A_par x.  Replaces CCPi::parallel_beam::f2D
+   (Core/src/instruments.hpp:266-271, Core/src/p2D.cpp:468-611). */
+int ccpi_pb_f2D(const double *h_pixels, const double *v_pixels,
+                const double *angles, int n_angles, int n_h, int n_v,
+                const double vox_origin[3], const double vox_size[3],
+                int nx, int ny, int nz, float *pixels, const float *voxels,
+                void *stream);
+
+/* x += A_par^T y.  Replaces CCPi::parallel_beam::b2D
+   (Core/src/instruments.hpp:272-277, Core/src/p2D.cpp:950-1087). */
+int ccpi_pb_b2D(const double *h_pixels, const double *v_pixels,
+                const double *angles, const float *pixels, float *voxels,
+                int n_angles, int n_h, int n_v, const double vox_origin[3],
+                const double vox_size[3], int nx, int ny, int nz,
+                void *stream);
+
+/* y = (y + A_cone x) * d_conv.  Replaces CCPi::cone_beam::f2D
+   (Core/src/instruments.hpp:164-171, Core/src/c2D.cpp:469-625). */
+int ccpi_cb_f2D(double source_x, double source_y, double source_z,
+                double detector_x, const double *h_pixels,
+                const double *v_pixels, const double *angles, float *pixels,
+                const float *voxels, int n_angles, int n_h, int n_v,
+                const double vox_origin[3], const double vox_size[3],
+                int nx, int ny, int nz, void *stream);
+
+/* x += A_cone^T (y * d_conv), y untouched (limited_memory = false).
+   Replaces CCPi::cone_beam::b2D (Core/src/instruments.hpp:172-179,
+   Core/src/c2D.cpp:948-1119). */
+int ccpi_cb_b2D(double source_x, double source_y, double source_z,
+                double detector_x, const double *h_pixels,
+                const double *v_pixels, const double *angles,
+                const float *pixels, float *voxels, int n_angles, int n_h,
+                int n_v, const double vox_origin[3],
+                const double vox_size[3], int nx, int ny, int nz,
+                void *stream);
+
+/* Same projectors on a geometry descriptor (the instrument's virtual
+   forward_project / backward_project, Core/src/instruments.hpp:61-69). */
+int ccpi_forward_project(const ccpi_geometry *g, float *pixels,
+                         const float *voxels, void *stream);
+int ccpi_backward_project(const ccpi_geometry *g, const float *pixels,
+                          float *voxels, void *stream);
+
+/* ---- iteration bodies --------------------------------------------------
+   reconstruction_alg::reconstruct (Core/src/algorithms.hpp:13-21).
+   b: measured sinogram (device); x: volume (device, in/out). */
+
+/* CGLS, cgls_3d (Core/src/Algorithms/cgls.cpp:26-157).  b is overwritten
+   with the final residual and x receives x + CGLS(b), as the reference.
+   norms (host, may be NULL) receives ||A^T r|| after each iteration. */
+int ccpi_cgls(const ccpi_geometry *g, float *b, float *x, int iterations,
+              double *norms, int flags, void *stream);
+
+/* SIRT (Core/src/Algorithms/sirt.cpp:15-80); x is the starting point. */
+int ccpi_sirt(const ccpi_geometry *g, const float *b, float *x,
+              int iterations, int flags, void *stream);
+
+/* MLEM (Core/src/Algorithms/mlem.cpp:15-80); x is reset to 1 first. */
+int ccpi_mlem(const ccpi_geometry *g, const float *b, float *x,
+              int iterations, int flags, void *stream);
+
+/* ---- batched parallel-beam CGLS ----------------------------------------
+   No counterpart in the reference (one problem per call there).  Runs
+   `count` independent parallel-beam CGLS problems of uneven sizes in one
+   persistent launch per step, captured in a hipGraph.  b[i] / x[i] are
+   device pointers per problem; norms (host, may be NULL) is
+   [count][iterations]. */
+int ccpi_pb_cgls_batched(const ccpi_geometry *geoms, int count,
+                         float *const *b, float *const *x, int iterations,
+                         double *norms, int flags, void *stream);
+
+/* ---- utilities --------------------------------------------------------- */
+const char *ccpi_error_string(int status);
+/* Detail of the last failing call on this host thread (empty after a
+   successful call). */
+const char *ccpi_last_error(void);
+int ccpi_version(void);
+/* Drop cached per-geometry plans (device tables) on the current device. */
+void ccpi_plan_cache_clear(void);
+/* Device bytes held by cached plans on the current device. */
+long long ccpi_plan_cache_bytes(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CCPI_HIP_H */

@@ -224,6 +224,7 @@ void FN(or_pb_f2D)(const double *h_pix, int nh, const double *v_pix, int nv,
             else         { ymin = wy * c - vx * s; ymax = vy * c - wx * s; }
           }
           hmin = ymin < h_pix[0] ? 0 : (int)floor(ymin * ihp - hp0);
+          if (hmin < 0) hmin = 0;   /* the floor can round to -1 */
           hmax = (int)floor(ymax * ihp - hp0);
           if (hmax > nh - 1) hmax = nh - 1;
         }
@@ -336,6 +337,7 @@ void FN(or_pb_b2D)(const double *h_pix, int nh, const double *v_pix, int nv,
             else         { ymin = yn * c - x0 * s; ymax = y0 * c - xn * s; }
           }
           lo = ymin < h_pix[0] ? 0 : (int)floor(ymin * ihp - hp0);
+          if (lo < 0) lo = 0;   /* the floor can round to -1 */
           hi = (int)floor(ymax * ihp - hp0);
           if (hi > nh - 1) hi = nh - 1;
         }

@@ -1,0 +1,80 @@
+"""CPU checks of the C ABI: the library loads, exports every function
+include/ccpi_hip.h declares, and rejects invalid geometry with status codes
+before touching a device (no compute calls here)."""
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ccpi.reconstruction import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "ccpi_hip.h")).read()
+    return sorted(set(re.findall(r"^\w[\w \*]*?\b(ccpi_\w+)\s*\(", text, re.M)))
+
+
+def test_header_and_binding_agree():
+    assert sorted(_lib.EXPORTS) == _declared()
+
+
+def test_library_exports_every_symbol():
+    L = _lib.lib()
+    for name in _declared():
+        assert hasattr(L, name), name
+
+
+def test_utilities_without_device():
+    L = _lib.lib()
+    assert L.ccpi_version() >= 100
+    assert L.ccpi_error_string(-2).decode().startswith("geometry")
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+@pytest.mark.parametrize("case", ["one_column", "decreasing_h", "zero_voxel"])
+def test_parallel_geometry_rejected(case):
+    L = _lib.lib()
+    h = np.array([-1.0, 0.0, 1.0])
+    v = np.array([0.5])
+    ang = np.array([0.0, 0.5])
+    o = np.array([-1.0, -1.0, 0.0])
+    s = np.array([1.0, 1.0, 1.0])
+    if case == "one_column":
+        h = h[:1]
+    elif case == "decreasing_h":
+        h = h[::-1].copy()
+    else:
+        s = np.array([1.0, 0.0, 1.0])
+    dummy = C.c_void_p(16)  # never dereferenced: validation fails first
+    st = L.ccpi_pb_f2D(_dp(h), _dp(v), _dp(ang), 2, len(h), 1, _dp(o), _dp(s), 2, 2, 1,
+                       dummy, dummy, None)
+    assert st == -2
+    assert L.ccpi_last_error()
+
+
+def test_cone_geometry_rejected():
+    """A detector row at the source height (the reference's 'Oops',
+    c2D.cpp:487-492) and nz < 2 (b2D divides by nz - 1, c2D.cpp:808)."""
+    L = _lib.lib()
+    h = np.array([-1.0, 1.0])
+    ang = np.array([0.0])
+    o = np.array([-2.0, -2.0, -2.0])
+    s = np.array([1.0, 1.0, 1.0])
+    dummy = C.c_void_p(16)
+    v = np.array([-1.0, 0.0, 1.0])
+    assert L.ccpi_cb_f2D(-50.0, 0.0, 0.0, 50.0, _dp(h), _dp(v), _dp(ang), dummy, dummy,
+                         1, 2, 3, _dp(o), _dp(s), 4, 4, 4, None) == -2
+    v = np.array([-1.0, 1.0])
+    assert L.ccpi_cb_b2D(-50.0, 0.0, 0.0, 50.0, _dp(h), _dp(v), _dp(ang), dummy, dummy,
+                         1, 2, 2, _dp(o), _dp(s), 4, 4, 1, None) == -2
+    # source inside the volume's rotation circle
+    assert L.ccpi_cb_f2D(-2.0, 0.0, 0.0, 50.0, _dp(h), _dp(v), _dp(ang), dummy, dummy,
+                         1, 2, 2, _dp(o), _dp(s), 4, 4, 4, None) == -2
