@@ -1,0 +1,125 @@
+"""``ccpi.reconstruction.conebeam``: drop-in for the reference module of the
+same name (Wrappers/Python/src/conebeam_module.cpp:23-42,
+conebeam_wrapper.cpp).
+
+Behaviour kept from the reference path (conebeam_wrapper.cpp:14-121,
+voxels.cpp:208-290):
+  * scans are [angle][h][v] float32, angles in degrees;
+  * source_x = -source_to_sample, detector_x = source_to_detector +
+    source_x; XTek geometry and voxel size from the mask radius
+    (xtek.cpp:57-129, 760-798);
+  * -log unless is_pixels_in_log, then the find_centre search shifts the
+    source and h pixels (xtek.cpp:563, 671-758);
+  * the result is clamped at 0; nthreads is ignored.
+The reference overwrites its full_vox_origin / voxel_size arguments in C++
+only; here they are returned by ``alg.last_geometry()``.
+``create_sinogram`` is the static cone f2D with explicit geometry and angles
+in radians (conebeam_wrapper.cpp:346-400).
+"""
+
+from __future__ import annotations
+
+import types
+
+import numpy as np
+
+from . import instruments as _ins
+from . import projectors as _pj
+
+alg = types.ModuleType("ccpi.reconstruction.conebeam.alg")
+_last = {}
+
+
+def _device():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("ccpi.reconstruction needs a HIP device")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _reconstruct(algo, pixels, angles, h_offsets, v_offsets, pixels_per_voxel,
+                 source_to_sample, source_to_detector, pixel_h_size, pixel_v_size,
+                 mask_radius, beam_harden, full_vox_origin, voxel_size, niterations,
+                 nthreads, is_pixels_in_log):
+    import torch
+    dev = _device()
+    pix = np.ascontiguousarray(np.asarray(pixels, dtype=np.float32))
+    if pix.ndim != 3:
+        raise ValueError("pixels must be [angle][h][v]")
+    sx = -float(source_to_sample)
+    dx = float(source_to_detector) + sx
+    setup = _ins.xtek_setup(pix.shape, angles, int(pixels_per_voxel), sx, dx,
+                            float(pixel_h_size), float(pixel_v_size), float(mask_radius))
+    b = _ins.xtek_read_scans(torch.from_numpy(pix).to(dev), setup, bool(is_pixels_in_log))
+    g = setup.geometry
+    _ins.find_centre(setup, b[:, :, g.v_pixels.size // 2].cpu().numpy())
+    x = torch.zeros(g.shape, dtype=torch.float32, device=dev)
+    if algo == "cgls":
+        x, _, norms = _pj.cgls(g, b, int(niterations), x)
+        _last["norms"] = norms
+    elif algo == "sirt":
+        x = _pj.sirt(g, b, int(niterations), x)
+    else:
+        x = _pj.mlem(g, b, int(niterations), x)
+    x.clamp_(min=0.0)
+    _last["geometry"] = g
+    return x.cpu().numpy()
+
+
+def cgls(pixels, angles, h_offsets, v_offsets, pixels_per_voxel, source_to_sample,
+         source_to_detector, pixel_h_size, pixel_v_size, mask_radius, beam_harden,
+         full_vox_origin, voxel_size, niterations, nthreads, is_pixels_in_log=True):
+    """CGLS (conebeam_wrapper.cpp:124-138)."""
+    return _reconstruct("cgls", pixels, angles, h_offsets, v_offsets, pixels_per_voxel,
+                        source_to_sample, source_to_detector, pixel_h_size, pixel_v_size,
+                        mask_radius, beam_harden, full_vox_origin, voxel_size, niterations,
+                        nthreads, is_pixels_in_log)
+
+
+def sirt(pixels, angles, h_offsets, v_offsets, pixels_per_voxel, source_to_sample,
+         source_to_detector, pixel_h_size, pixel_v_size, mask_radius, beam_harden,
+         full_vox_origin, voxel_size, niterations, nthreads, is_pixels_in_log=True):
+    """SIRT (conebeam_wrapper.cpp:140-154)."""
+    return _reconstruct("sirt", pixels, angles, h_offsets, v_offsets, pixels_per_voxel,
+                        source_to_sample, source_to_detector, pixel_h_size, pixel_v_size,
+                        mask_radius, beam_harden, full_vox_origin, voxel_size, niterations,
+                        nthreads, is_pixels_in_log)
+
+
+def mlem(pixels, angles, h_offsets, v_offsets, pixels_per_voxel, source_to_sample,
+         source_to_detector, pixel_h_size, pixel_v_size, mask_radius, beam_harden,
+         full_vox_origin, voxel_size, niterations, nthreads, is_pixels_in_log=True):
+    """MLEM (conebeam_wrapper.cpp:156-170)."""
+    return _reconstruct("mlem", pixels, angles, h_offsets, v_offsets, pixels_per_voxel,
+                        source_to_sample, source_to_detector, pixel_h_size, pixel_v_size,
+                        mask_radius, beam_harden, full_vox_origin, voxel_size, niterations,
+                        nthreads, is_pixels_in_log)
+
+
+def last_geometry():
+    """Geometry (with the find_centre shift) of the last reconstruction."""
+    return _last.get("geometry")
+
+
+def cgls_norms():
+    return np.array(_last.get("norms", np.zeros(0)))
+
+
+def create_sinogram(source_x, source_y, source_z, detector_x, h_pixels, v_pixels, angles,
+                    voxels, grid_offset, voxel_size):
+    """Static cone f2D with explicit geometry; angles in radians;
+    returns [angle][h][v] (conebeam_wrapper.cpp:346-400)."""
+    import torch
+    dev = _device()
+    vol = np.ascontiguousarray(np.asarray(voxels, dtype=np.float32))
+    f = lambda a: np.asarray(a, dtype=np.float32).astype(np.float64)
+    g = _pj.Geometry(f(angles), f(h_pixels), f(v_pixels), f(grid_offset)[:3],
+                     f(voxel_size)[:3], vol.shape, cone=True,
+                     source=(float(np.float32(source_x)), float(np.float32(source_y)),
+                             float(np.float32(source_z))),
+                     detector_x=float(np.float32(detector_x)))
+    return _pj.forward_project(g, torch.from_numpy(vol).to(dev)).cpu().numpy()
+
+
+for _f in (cgls, sirt, mlem, last_geometry, cgls_norms, create_sinogram):
+    setattr(alg, _f.__name__, _f)

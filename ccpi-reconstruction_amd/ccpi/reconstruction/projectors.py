@@ -17,7 +17,8 @@ from typing import Sequence
 
 import numpy as np
 
-from ._lib import CCPI_CONE, CCPI_PARALLEL, CCPI_REDUCE_FP32, CcpiGeometry, check, lib
+from ._lib import (CCPI_CONE, CCPI_PARALLEL, CCPI_REDUCE_FP32, CCPI_TIMING, CcpiGeometry,
+                   check, lib)
 
 
 def _f64(a) -> np.ndarray:
@@ -170,6 +171,58 @@ def cgls_batched(geoms: Sequence[Geometry], bs, iterations: int, xs=None,
                                      norms.ctypes.data_as(C.POINTER(C.c_double)),
                                      CCPI_REDUCE_FP32 if reduce_fp32 else 0, _stream()))
     return xs, rs, norms[:, :iterations]
+
+
+class CglsSession:
+    """CGLS split into launch-only steps (C ABI ccpi_cgls_create/iterate):
+    iterate() enqueues work on the current stream without host sync.  The
+    session holds ``b`` (overwritten with the residual) and ``x``."""
+
+    def __init__(self, geom: Geometry, b, x, max_iterations: int, reduce_fp32=False,
+                 timing=False):
+        self.geom, self.b, self.x = geom, b, x
+        flags = (CCPI_REDUCE_FP32 if reduce_fp32 else 0) | (CCPI_TIMING if timing else 0)
+        h = C.c_void_p()
+        check(lib().ccpi_cgls_create(C.byref(geom.c_struct()), _dev(b, geom.sino_shape, "b"),
+                                     _dev(x, geom.shape, "x"), int(max_iterations), flags,
+                                     _stream(), C.byref(h)))
+        self._h = h
+
+    def iterate(self, n: int = 1) -> None:
+        check(lib().ccpi_cgls_iterate(self._h, int(n)))
+
+    def norms(self) -> np.ndarray:
+        buf = np.zeros(4096, np.float64)
+        done = C.c_int()
+        check(lib().ccpi_cgls_norms(self._h, buf.ctypes.data_as(C.POINTER(C.c_double)),
+                                    C.byref(done)))
+        return buf[:done.value].copy()
+
+    def timings(self):
+        """(fp_ms_total, fp_count, bp_ms_total, bp_count) from HIP events."""
+        fpm, bpm = C.c_double(), C.c_double()
+        fpc, bpc = C.c_int(), C.c_int()
+        check(lib().ccpi_cgls_timings(self._h, C.byref(fpm), C.byref(fpc), C.byref(bpm),
+                                      C.byref(bpc)))
+        return fpm.value, fpc.value, bpm.value, bpc.value
+
+    def close(self) -> None:
+        if self._h:
+            check(lib().ccpi_cgls_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def projector_nnz(geom: Geometry) -> int:
+    """Nonzero system-matrix entries of the projector, counted on device."""
+    n = C.c_ulonglong()
+    check(lib().ccpi_projector_nnz(C.byref(geom.c_struct()), C.byref(n), _stream()))
+    return int(n.value)
 
 
 def plan_cache_clear() -> None:

@@ -151,6 +151,93 @@ int ccpi_mlem(const ccpi_geometry *g, const float *b, float *x, int iterations, 
   });
 }
 
+}  // extern "C"
+
+// Session state behind the opaque ccpi_cgls_state handle.
+struct ccpi_cgls_state {
+  ccpi::Projector P;
+  ccpi::Workspace W;
+  ccpi::EventLog log;
+  ccpi::CglsRun run;
+  hipStream_t st;
+  int max_iters;
+  int done = 0;
+  ccpi_cgls_state(const ccpi::Geometry &G, float *b, float *x, int n, int flags,
+                  hipStream_t s)
+      : P(G, s), run{P, W, b, x, n, (flags & CCPI_REDUCE_FP32) != 0}, st(s), max_iters(n) {
+    if (flags & CCPI_TIMING) run.log = &log;
+  }
+};
+
+extern "C" {
+
+int ccpi_cgls_create(const ccpi_geometry *g, float *b, float *x, int max_iterations,
+                     int flags, void *stream, ccpi_cgls_state **state) {
+  return guarded([&] {
+    CCPI_REQUIRE(b && x && state, CCPI_E_ARG, "null pointer");
+    CCPI_REQUIRE(max_iterations >= 0, CCPI_E_ARG, "negative iteration count");
+    *state = nullptr;
+    auto *s = new ccpi_cgls_state(ccpi::Geometry::from(g), b, x, max_iterations, flags,
+                                  S(stream));
+    try {
+      s->run.start(s->st);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *state = s;
+  });
+}
+
+int ccpi_cgls_iterate(ccpi_cgls_state *s, int n) {
+  return guarded([&] {
+    CCPI_REQUIRE(s && n >= 0, CCPI_E_ARG, "bad session call");
+    CCPI_REQUIRE(s->done + n <= s->max_iters, CCPI_E_LIMIT,
+                 "more iterations than the session was created for");
+    for (int q = 0; q < n; q++) s->run.iterate(s->done++, s->st);
+  });
+}
+
+int ccpi_cgls_norms(ccpi_cgls_state *s, double *norms, int *done) {
+  return guarded([&] {
+    CCPI_REQUIRE(s, CCPI_E_ARG, "null session");
+    if (norms && s->done > 0) {
+      CCPI_HIP(hipMemcpyAsync(norms, s->run.norms_dev.p, (size_t)s->done * sizeof(double),
+                              hipMemcpyDeviceToHost, s->st));
+    }
+    CCPI_HIP(hipStreamSynchronize(s->st));
+    if (done) *done = s->done;
+  });
+}
+
+int ccpi_cgls_timings(ccpi_cgls_state *s, double *fp_ms, int *fp_count, double *bp_ms,
+                      int *bp_count) {
+  return guarded([&] {
+    CCPI_REQUIRE(s, CCPI_E_ARG, "null session");
+    CCPI_HIP(hipStreamSynchronize(s->st));
+    if (fp_ms) *fp_ms = ccpi::EventLog::total_ms(s->log.fp);
+    if (bp_ms) *bp_ms = ccpi::EventLog::total_ms(s->log.bp);
+    if (fp_count) *fp_count = (int)(s->log.fp.size() / 2);
+    if (bp_count) *bp_count = (int)(s->log.bp.size() / 2);
+  });
+}
+
+int ccpi_cgls_destroy(ccpi_cgls_state *s) {
+  return guarded([&] {
+    if (!s) return;
+    (void)hipStreamSynchronize(s->st);
+    delete s;
+  });
+}
+
+int ccpi_projector_nnz(const ccpi_geometry *g, unsigned long long *nnz, void *stream) {
+  return guarded([&] {
+    CCPI_REQUIRE(nnz, CCPI_E_ARG, "null output");
+    ccpi::Projector P(ccpi::Geometry::from(g), S(stream));
+    *nnz = ccpi::projector_nnz(P, S(stream));
+  });
+}
+
 int ccpi_pb_cgls_batched(const ccpi_geometry *geoms, int count, float *const *b,
                          float *const *x, int iterations, double *norms, int flags,
                          void *stream) {

@@ -265,6 +265,46 @@ __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
   *out = acc * g.dconv[(size_t)h * g.nv + v];
 }
 
+// Counts the nonzero-weight voxel operands of cone_fp_kernel (same row
+// rules), i.e. nnz of the system matrix.
+__global__ __launch_bounds__(kFpRows) void cone_nnz_kernel(
+    ConeArgs g, const ConeRay *__restrict__ rays, const ConeEnt *__restrict__ ents,
+    unsigned long long *__restrict__ count) {
+  extern __shared__ float s_vz[];
+  __shared__ unsigned long long s_tot;
+  for (int q = threadIdx.x; q <= g.nz; q += kFpRows) s_vz[q] = g.vz[q];
+  if (threadIdx.x == 0) s_tot = 0;
+  __syncthreads();
+  const int r = blockIdx.x;
+  const int v = blockIdx.y * kFpRows + threadIdx.x;
+  const ConeRay R = rays[r];
+  unsigned long long n = 0;
+  if (v < g.nv && R.n > 1) {
+    const float dzv = g.dz[v], izv = g.iz[v];
+    const bool lower = v < g.mid;
+    const int dir = lower ? -1 : 1;
+    const int nz = g.nz;
+    const ConeEnt *__restrict__ e = ents + R.off;
+    float a0 = e[0].alpha;
+    for (int m = 1; m < R.n; ++m) {
+      const float a1 = e[m].alpha;
+      const float ai = a0 * g.inv_dz;
+      const bool edge = (m >= R.ncom) && (v < R.min_v);
+      const int k = (int)((edge ? g.pzbz1 : g.pzbz) + ai * dzv) - (edge ? 1 : 0);
+      const int kz = min(max(lower ? k : k + 1, 0), nz);
+      const float az = s_vz[kz] * izv;
+      const float mz = a1 < az ? a1 : az;
+      const int k2 = k + dir;
+      n += (k >= 0 && k < nz && mz - a0 != 0.f) ? 1 : 0;
+      n += (k2 >= 0 && k2 < nz && a1 - mz != 0.f) ? 1 : 0;
+      a0 = a1;
+    }
+  }
+  atomicAdd(&s_tot, n);
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(count, s_tot);
+}
+
 // ---------------------------------------------------------------------------
 // backprojection
 // ---------------------------------------------------------------------------
@@ -421,40 +461,71 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
       }
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      // sweep rows for every entry of the batch
+      // sweep the batch: row block outer, entries inner, so each lane's row
+      // constants are loaded once per block.  Lane L takes rows
+      // vb + S*L + t (t < S): with S at least the largest number of rows
+      // per slice, the lanes of one instruction hit distinct slices and the
+      // column update is a plain LDS read-add-write.  Slice indices are
+      // non-decreasing across lanes, so a duplicate can only sit in the
+      // next lane; one DPP shift detects it and that instruction falls back
+      // to ds_add.
+      const int S = g.bp_stride;
       const int npass = one_set ? 1 : 4;
-      for (int e = 0; e < used; ++e) {
-        const uint32_t ray = e_ray[e];
-        const float a0 = e_a0[e], a1 = e_a1[e];
-        const float ai = a0 * g.inv_dz;
-        const float *__restrict__ prow = dpix + (size_t)ray * nv;
-        for (int vb = 0; vb < nv; vb += 64) {
-          const int v = vb + lane;
-          if (v >= nv) break;
-          const float p = prow[v];
-          const float dzv = g.dz[v], izv = g.iz[v];
+      for (int vb = 0; vb < nv; vb += 64 * S) {
+        for (int t = 0; t < S; ++t) {
+          const int v = vb + S * lane + t;
+          const bool live = v < nv;
+          const int vv = live ? v : nv - 1;
+          const float dzv = g.dz[vv], izv = g.iz[vv];
           for (int pass = 0; pass < npass; ++pass) {
             // row set: 0 lower edge, 1 lower main, 2 upper main, 3 upper edge
             int set;
+            bool member;
             if (one_set) {
               set = v < min_v ? 0 : (v < mid ? 1 : (v < max_v ? 2 : 3));
+              member = live;
             } else {
               set = pass;
-              const bool member = (set == 0) ? (v < min_v)
-                                : (set == 1) ? (v >= min_v && v < mid)
-                                : (set == 2) ? (v >= mid && v < max_v)
-                                             : (v >= max_v);
-              if (!member) continue;
+              member = live && ((set == 0) ? (v < min_v)
+                              : (set == 1) ? (v >= min_v && v < mid)
+                              : (set == 2) ? (v >= mid && v < max_v)
+                                           : (v >= max_v));
             }
+            if (!__any(member)) continue;
             const bool lower = set <= 1;
-            const int k = (set == 0) ? (int)(g.pzbz1 + ai * dzv) - 1
-                                     : (int)(g.pzbz + ai * dzv);
-            const int kz = min(max(lower ? k : k + 1, 0), nz);
-            const float az = s_vz[kz] * izv;
-            const float mz = a1 < az ? a1 : az;
-            const int k2 = lower ? k - 1 : k + 1;
-            if (k >= 0 && k < nz) atomicAdd(&acc[k], p * (mz - a0));
-            if (k2 >= 0 && k2 < nz) atomicAdd(&acc[k2], p * (a1 - mz));
+            const float kbase = set == 0 ? g.pzbz1 : g.pzbz;
+            const int ksub = set == 0 ? 1 : 0;
+            const int kzoff = lower ? 0 : 1;
+            const int dir = lower ? -1 : 1;
+            const float *__restrict__ pcol = dpix + v;
+            for (int e = 0; e < used; ++e) {
+              const uint32_t ray = e_ray[e];
+              const float a0 = e_a0[e], a1 = e_a1[e];
+              const float ai = a0 * g.inv_dz;
+              const int k = (int)(kbase + ai * dzv) - ksub;
+              const int k2 = k + dir;
+              const bool ok0 = member && k >= 0 && k < nz;
+              const bool ok1 = member && k2 >= 0 && k2 < nz;
+              float p = 0.f;
+              if (member) p = pcol[(size_t)ray * nv];
+              const int kz = min(max(k + kzoff, 0), nz);
+              const float az = s_vz[kz] * izv;
+              const float mz = a1 < az ? a1 : az;
+              const float c0 = p * (mz - a0), c1 = p * (a1 - mz);
+              // unique negative keys for idle lanes never compare equal
+              const int key0 = ok0 ? k : -1 - lane, key1 = ok1 ? k2 : -1 - lane;
+              const int prev0 = __builtin_amdgcn_update_dpp(0x7fffffff, key0, 0x138, 0xf, 0xf, false);
+              const int prev1 = __builtin_amdgcn_update_dpp(0x7fffffff, key1, 0x138, 0xf, 0xf, false);
+              if (__any(key0 == prev0 || key1 == prev1)) {
+                if (ok0) atomicAdd(&acc[k], c0);
+                if (ok1) atomicAdd(&acc[k2], c1);
+              } else {
+                if (ok0) acc[k] = acc[k] + c0;
+                __builtin_amdgcn_wave_barrier();
+                if (ok1) acc[k2] = acc[k2] + c1;
+                __builtin_amdgcn_wave_barrier();
+              }
+            }
           }
         }
       }
@@ -560,6 +631,25 @@ void cone_build_plan(ConePlan &P, hipStream_t st) {
     hr[2 * a] = std::max(std::min(std::min(h00, h01), std::min(h10, h11)), 0);
     hr[2 * a + 1] = std::min(std::max(std::max(h00, h01), std::max(h10, h11)), nh - 1);
   }
+  // BP row stride: at least the largest number of detector rows per slice
+  // over all columns.  The walk parameter of any voxel is at least
+  // (|source| - R) / L, R the farthest voxel from the axis and L the longest
+  // source -> pixel xy distance.
+  {
+    double R = 0.0;
+    for (int cx = 0; cx < 2; cx++)
+      for (int cy = 0; cy < 2; cy++)
+        R = std::max(R, std::hypot(G.origin[0] + cx * nx * G.vsize[0],
+                                   G.origin[1] + cy * ny * G.vsize[1]));
+    double Lmax = 0.0;
+    for (int h : {0, nh - 1})
+      Lmax = std::max(Lmax, std::hypot(G.det_x - sx, G.h_pix[h] - sy));
+    const double amin = std::max((std::hypot(sx, sy) - R) / Lmax, 1e-3);
+    double pitch = 1e300;
+    for (int v = 1; v < nv; v++) pitch = std::min(pitch, G.v_pix[v] - G.v_pix[v - 1]);
+    const double rows_per_slice = G.vsize[2] / (amin * pitch);
+    A.bp_stride = std::max(1, std::min(16, (int)std::ceil(rows_per_slice * 1.05)));
+  }
   P.atab.upload(atab, st);
   P.hpix.upload(G.h_pix, st);
   P.hrange.upload(hr, st);
@@ -612,6 +702,15 @@ void cone_fp(const ConePlan &P, const float *vol, float *sino, hipStream_t st) {
   const size_t lds = sizeof(float) * (size_t)(A.nz + 1);
   hipLaunchKernelGGL(cone_fp_kernel, grid, dim3(kFpRows), lds, st, A,
                      P.rays.as<ConeRay>(), P.ents.as<ConeEnt>(), vol, sino);
+  CCPI_HIP(hipGetLastError());
+}
+
+void cone_nnz(const ConePlan &P, unsigned long long *count, hipStream_t st) {
+  const ConeArgs &A = P.args;
+  dim3 grid((unsigned)(A.na * A.nh), (unsigned)((A.nv + kFpRows - 1) / kFpRows));
+  const size_t lds = sizeof(float) * (size_t)(A.nz + 1);
+  hipLaunchKernelGGL(cone_nnz_kernel, grid, dim3(kFpRows), lds, st, A,
+                     P.rays.as<ConeRay>(), P.ents.as<ConeEnt>(), count);
   CCPI_HIP(hipGetLastError());
 }
 

@@ -251,6 +251,71 @@ __global__ __launch_bounds__(256) void par_bp_kernel(ParArgs g,
     if (q < nk) vcol[q] = acc[q];
 }
 
+// Nonzero system-matrix entries: per ray, crossed cells with nonzero xy
+// length times the detector rows that map into the volume.
+__global__ __launch_bounds__(256) void par_nnz_kernel(ParArgs g,
+                                                      unsigned long long *__restrict__ count) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long n = 0;
+  if (r < g.na * g.nh) {
+    const int a = r / g.nh, h = r - a * g.nh;
+    int rows = 0;
+    for (int v = 0; v < g.nv; v++) rows += (g.map[v] >= 0 && g.map[v] < g.nz) ? 1 : 0;
+    const double c = g.atab[5 * a], s = g.atab[5 * a + 1];
+    const double p2x = c * g.D - s * g.h_pix[h];
+    const double p2y = s * g.D + c * g.h_pix[h];
+    int cells = 0;
+    if (fabs(c) < kEps) {
+      const int i = (int)floor((p2x - g.ox) / g.dx);
+      if (fabs(s) >= kEps && i >= 0 && i < g.nx) cells = g.ny;
+    } else if (fabs(s) < kEps) {
+      const int j = (int)floor((p2y - g.oy) / g.dy);
+      if (j >= 0 && j < g.ny) cells = g.nx;
+    } else {
+      const double ix = 1.0 / c, iy = 1.0 / s;
+      const double ax0 = (g.ox - p2x) * ix, ay0 = (g.oy - p2y) * iy;
+      const double axn = (g.ox + (double)g.nx * g.dx - p2x) * ix;
+      const double ayn = (g.oy + (double)g.ny * g.dy - p2y) * iy;
+      const double amin = fmax(fmax(fmin(ax0, axn), fmin(ay0, ayn)), -g.dconv);
+      const double amax = fmin(fmin(fmax(ax0, axn), fmax(ay0, ayn)), 0.0);
+      if (amin < amax - kEps) {
+        // |cells crossed| = steps until the walk leaves the grid
+        const int sx = c > 0.0 ? 1 : -1, sy = s > 0.0 ? 1 : -1;
+        int x = (int)floor((p2x + amin * c - g.ox) / g.dx);
+        int y = (int)floor((p2y + amin * s - g.oy) / g.dy);
+        x = min(max(x, 0), g.nx - 1);
+        y = min(max(y, 0), g.ny - 1);
+        const int bxs = sx > 0, bys = sy > 0;
+        double ap = amin;
+        while (x >= 0 && x < g.nx && y >= 0 && y < g.ny) {
+          const double nxa = (g.ox + (double)(x + bxs) * g.dx - p2x) * ix;
+          const double nya = (g.oy + (double)(y + bys) * g.dy - p2y) * iy;
+          double nap;
+          if (nxa < nya - kEps) { nap = nxa; x += sx; }
+          else if (nya < nxa - kEps) { nap = nya; y += sy; }
+          else { nap = fmax(nxa, nya); x += sx; y += sy; }
+          cells += ((float)(nap - ap) != 0.f) ? 1 : 0;
+          ap = nap;
+        }
+      }
+    }
+    n = (unsigned long long)cells * (unsigned long long)rows;
+  }
+  __shared__ unsigned long long s_tot;
+  if (threadIdx.x == 0) s_tot = 0;
+  __syncthreads();
+  atomicAdd(&s_tot, n);
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(count, s_tot);
+}
+
+void par_nnz(const ParPlan &P, unsigned long long *count, hipStream_t st) {
+  const ParArgs &A = P.args;
+  const int rays = A.na * A.nh;
+  hipLaunchKernelGGL(par_nnz_kernel, dim3((rays + 255) / 256), dim3(256), 0, st, A, count);
+  CCPI_HIP(hipGetLastError());
+}
+
 void par_build_plan(ParPlan &P, hipStream_t st) {
   const Geometry &G = P.g;
   ParArgs A{};

@@ -114,6 +114,9 @@ size_t cone_bp_lds_bytes(int nz);
 void par_build_plan(ParPlan &P, hipStream_t st);
 void par_fp(const ParPlan &P, const float *vol, float *sino, hipStream_t st);
 void par_bp(const ParPlan &P, const float *sino, float *vol, hipStream_t st);
+// nonzero system-matrix entries (gathered voxel operands of one FP)
+void cone_nnz(const ConePlan &P, unsigned long long *count, hipStream_t st);
+void par_nnz(const ParPlan &P, unsigned long long *count, hipStream_t st);
 
 // Projector pair on one geometry with device-resident plan; BP leaves the
 // sinogram untouched.

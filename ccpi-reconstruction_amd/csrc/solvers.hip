@@ -248,12 +248,37 @@ void CglsRun::start(hipStream_t st) {
   CCPI_HIP(hipGetLastError());
 }
 
+EventLog::~EventLog() {
+  for (hipEvent_t e : fp) (void)hipEventDestroy(e);
+  for (hipEvent_t e : bp) (void)hipEventDestroy(e);
+}
+
+void EventLog::mark(std::vector<hipEvent_t> &v, hipStream_t st) {
+  hipEvent_t e;
+  CCPI_HIP(hipEventCreate(&e));
+  v.push_back(e);
+  CCPI_HIP(hipEventRecord(e, st));
+}
+
+double EventLog::total_ms(const std::vector<hipEvent_t> &v) {
+  double t = 0.0;
+  for (size_t q = 0; q + 1 < v.size(); q += 2) {
+    float ms = 0.f;
+    CCPI_HIP(hipEventSynchronize(v[q + 1]));
+    CCPI_HIP(hipEventElapsedTime(&ms, v[q], v[q + 1]));
+    t += ms;
+  }
+  return t;
+}
+
 void CglsRun::iterate(int it, hipStream_t st) {
   const size_t nvox = P.geometry().n_vox(), npix = P.geometry().n_pix();
   float *d = W.v1.as<float>(), *s = W.v2.as<float>(), *Ad = W.p1.as<float>();
   CglsScalars *sc = W.scal.as<CglsScalars>();
   dev_fill(Ad, npix, 0.f, st);
+  if (log) log->mark(log->fp, st);
   P.fp(d, Ad, st);
+  if (log) log->mark(log->fp, st);
   if (fp32) {
     sumsq<float>(Ad, npix, W.part.p, st);
     hipLaunchKernelGGL(cgls_alpha_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
@@ -266,7 +291,9 @@ void CglsRun::iterate(int it, hipStream_t st) {
   hipLaunchKernelGGL(cgls_update_xr_kernel, dim3(grid_for(std::max(nvox, npix))), dim3(256),
                      0, st, x, d, nvox, b, Ad, npix, sc);
   dev_fill(s, nvox, 0.f, st);
+  if (log) log->mark(log->bp, st);
   P.bp(b, s, st);
+  if (log) log->mark(log->bp, st);
   if (fp32) {
     sumsq<float>(s, nvox, W.part.p, st);
     hipLaunchKernelGGL(cgls_beta_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
@@ -347,6 +374,24 @@ void run_mlem(const Projector &P, const float *b, float *x, int iters, Workspace
                        nvox);
   }
   CCPI_HIP(hipGetLastError());
+}
+
+}  // namespace ccpi
+
+namespace ccpi {
+
+unsigned long long projector_nnz(const Projector &P, hipStream_t st) {
+  DeviceBuffer cnt;
+  cnt.alloc(sizeof(unsigned long long));
+  CCPI_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), st));
+  if (P.cone)
+    cone_nnz(*P.cone, cnt.as<unsigned long long>(), st);
+  else
+    par_nnz(*P.par, cnt.as<unsigned long long>(), st);
+  unsigned long long h = 0;
+  CCPI_HIP(hipMemcpyAsync(&h, cnt.p, sizeof h, hipMemcpyDeviceToHost, st));
+  CCPI_HIP(hipStreamSynchronize(st));
+  return h;
 }
 
 }  // namespace ccpi

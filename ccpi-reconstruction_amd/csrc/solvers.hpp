@@ -1,6 +1,8 @@
 // Device-resident iteration bodies (see solvers.hip).
 #pragma once
 
+#include <vector>
+
 #include "plan.hpp"
 
 namespace ccpi {
@@ -24,6 +26,14 @@ struct Workspace {
   void ensure(size_t nvox, size_t npix);
 };
 
+// HIP events bracketing projector launches (resolved on demand).
+struct EventLog {
+  std::vector<hipEvent_t> fp, bp;   // pairs: start, stop
+  ~EventLog();
+  void mark(std::vector<hipEvent_t> &v, hipStream_t st);
+  static double total_ms(const std::vector<hipEvent_t> &v);
+};
+
 // One CGLS run split in launch-only steps (no host sync inside
 // start / iterate), so a caller may capture iterate() in a graph.
 struct CglsRun {
@@ -34,6 +44,7 @@ struct CglsRun {
   int iters;
   bool fp32;
   DeviceBuffer norms_dev;
+  EventLog *log = nullptr;
   void start(hipStream_t st);
   void iterate(int it, hipStream_t st);
   void finish(double *norms, hipStream_t st);
@@ -46,5 +57,6 @@ void run_sirt(const Projector &P, const float *b, float *x, int iters,
               Workspace &W, hipStream_t st);
 void run_mlem(const Projector &P, const float *b, float *x, int iters,
               Workspace &W, hipStream_t st);
+unsigned long long projector_nnz(const Projector &P, hipStream_t st);
 
 }  // namespace ccpi

@@ -115,6 +115,33 @@ int ccpi_sirt(const ccpi_geometry *g, const float *b, float *x,
 int ccpi_mlem(const ccpi_geometry *g, const float *b, float *x,
               int iterations, int flags, void *stream);
 
+/* ---- CGLS session: the same body split into launch-only steps ----------
+   No counterpart in the reference (its loop is internal to
+   cgls_base::reconstruct, cgls.cpp:54-82).  For callers that interleave
+   iterations with their own work or time them; iterate() enqueues work on
+   the session's stream without any host synchronisation.
+   CCPI_TIMING brackets every projector launch with HIP events. */
+#define CCPI_TIMING          2
+
+typedef struct ccpi_cgls_state ccpi_cgls_state;
+
+int ccpi_cgls_create(const ccpi_geometry *g, float *b, float *x,
+                     int max_iterations, int flags, void *stream,
+                     ccpi_cgls_state **state);
+int ccpi_cgls_iterate(ccpi_cgls_state *state, int n);
+/* Host sync; norms[0..done) = ||A^T r|| per iteration, *done = count. */
+int ccpi_cgls_norms(ccpi_cgls_state *state, double *norms, int *done);
+/* Host sync; summed kernel milliseconds of the forward projections and of
+   the backprojections since creation (CCPI_TIMING), and their counts. */
+int ccpi_cgls_timings(ccpi_cgls_state *state, double *fp_ms, int *fp_count,
+                      double *bp_ms, int *bp_count);
+int ccpi_cgls_destroy(ccpi_cgls_state *state);
+
+/* Number of nonzero system-matrix entries of a geometry's projector (the
+   voxel operands one forward projection gathers), counted on the device. */
+int ccpi_projector_nnz(const ccpi_geometry *g, unsigned long long *nnz,
+                       void *stream);
+
 /* ---- batched parallel-beam CGLS ----------------------------------------
    No counterpart in the reference (one problem per call there).  Runs
    `count` independent parallel-beam CGLS problems of uneven sizes in one
