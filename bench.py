@@ -70,6 +70,74 @@ def cpu_baseline(geom, phantom_np, n_sample: int):
     }
 
 
+def batched_vs_plain(iters: int):
+    """Config 3: 64 uneven parallel-beam CGLS problems, batched (one
+    persistent launch per step, graph replay) vs the same problems one after
+    another through the one-problem path.  Per-iteration time is
+    (T(iters) - T(0)) / iters for both, so set-up is excluded alike."""
+    import numpy as np
+    import torch
+
+    from ccpi.reconstruction import workloads as W
+    from ccpi.reconstruction.projectors import CglsSession, cgls_batched, forward_project
+
+    geoms = W.batched_problems(64, seed=3)
+    phs, bs = [], []
+    for i, g in enumerate(geoms):
+        ph = W.ellipsoid_phantom(g, 6, seed=300 + i, dens=(0.1, 1.0))
+        b = W.gaussian_noise(forward_project(g, ph), 0.01, seed=300 + i)
+        phs.append(ph)
+        bs.append(b)
+
+    def run_batched(n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xs, _, _ = cgls_batched(geoms, bs, n)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, xs
+
+    run_batched(1)                       # autotune + plans
+    t0b, _ = run_batched(0)
+    tb, xb = run_batched(iters)
+    per_b = (tb - t0b) / iters
+
+    sess = []
+    for g, b in zip(geoms, bs):
+        x = torch.zeros(g.shape, dtype=torch.float32, device=b.device)
+        sess.append((CglsSession(g, b.clone(), x, iters + 1), x))
+    torch.cuda.synchronize()
+    sess[0][0].iterate(1)                # warm the one-problem kernels
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for ss, _ in sess[1:]:
+        ss.iterate(iters)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sess[0][0].iterate(iters)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    per_p = (t1 - t0 + t2 - t1) / iters
+    rm_b, rm_p = [], []
+    for ph, x1, (ss, x2) in zip(phs, xb, sess):
+        rm_b.append(float(torch.sqrt(torch.mean((x1 - ph) ** 2))))
+    # the first session ran one extra iteration; compare the rest
+    for ph, (ss, x2) in zip(phs[1:], sess[1:]):
+        rm_p.append(float(torch.sqrt(torch.mean((x2 - ph) ** 2))))
+    rel = max(abs(a - b) / max(b, 1e-30) for a, b in zip(rm_b[1:], rm_p))
+    for ss, _ in sess:
+        ss.close()
+    return {
+        "workload": "64 parallel-beam CGLS problems, N in {128..512}, S in [1, 32], "
+                    "{90,180,360} angles, width ceil(sqrt2 N), 1% Gaussian noise, seed 3",
+        "iterations": iters,
+        "batched_iter_s": 1.0 / per_b,
+        "plain_iter_s": 1.0 / per_p,
+        "speedup": per_p / per_b,
+        "rmse_mean": float(np.mean(rm_b)),
+        "rmse_max_rel_diff_batched_vs_plain": rel,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +145,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=12,
                     help="angles in the CPU-oracle sample (0 disables it)")
+    ap.add_argument("--batched-iters", type=int, default=20,
+                    help="CGLS iterations of the batched-vs-plain comparison (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path, "
@@ -126,6 +196,11 @@ def main() -> int:
     rmse = float(torch.sqrt(torch.mean((x - phantom) ** 2)))
     log(f"bench: {secs / args.steps * 1e3:.1f} ms/iter, FP {fp_ms:.1f} ms, BP {bp_ms:.1f} ms")
 
+    batched = None
+    if args.batched_iters > 0:
+        log("bench: batched vs plain (64 parallel-beam problems)")
+        batched = batched_vs_plain(args.batched_iters)
+
     base = None
     if args.cpu_sample > 0:
         log("bench: CPU oracle sample")
@@ -166,6 +241,7 @@ def main() -> int:
         "bp_gbs": bytes_alg / (bp_ms * 1e-3) / 1e9,
         "rmse_vs_phantom": rmse,
         "norm_atr_last": float(norms[-1]) if len(norms) else None,
+        "batched": batched,
         "cpu_baseline": base,
     }
     print(json.dumps(line), flush=True)

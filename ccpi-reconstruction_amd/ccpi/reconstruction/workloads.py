@@ -105,3 +105,17 @@ def gaussian_noise(sino, rel_sigma: float, seed: int):
     g.manual_seed(seed)
     sigma = rel_sigma * float(sino.max())
     return (sino + sigma * torch.randn(sino.shape, generator=g, device=sino.device)).contiguous()
+
+
+def batched_problems(count: int = 64, seed: int = 3):
+    """Uneven parallel-beam problems: N in {128, 256, 384, 512},
+    S in [1, 32], angles in {90, 180, 360} over [0, pi), detector width
+    ceil(sqrt(2) N), 1 mm pixels (one geometry per problem)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.choice([128, 256, 384, 512]))
+        s = int(rng.integers(1, 33))
+        na = int(rng.choice([90, 180, 360]))
+        out.append(parallel_geometry(n, s, na, int(math.ceil(math.sqrt(2.0) * n))))
+    return out

@@ -1,0 +1,372 @@
+// Batched parallel-beam CGLS: many independent problems of uneven sizes
+// advanced together.
+//
+// No counterpart in the reference (it reconstructs one problem per call;
+// the iteration body is cgls_3d, Core/src/Algorithms/cgls.cpp:26-157).
+// Every step of an iteration is ONE persistent launch over a flat work list
+// that spans all problems:
+//   FP    items (problem, 256-ray block, row block)   -> Ad, partial ||Ad||^2
+//   alpha one workgroup per problem: fixed-order sum of its partials
+//   x/r   items (problem, element chunk)               -> x += a d, r -= a Ad
+//   BP    items (problem, 256-column block, slice block) -> s, partial ||s||^2
+//   beta  one workgroup per problem
+//   d     items (problem, element chunk)               -> d = s + b d
+// The six launches are captured once in a hipGraph and replayed per
+// iteration; scalars (gamma, alpha, beta, iteration index) live in the
+// problem descriptors in device memory, so a replay needs no host work.
+// The norms of the projector outputs are produced by the projector
+// epilogues (per work item, deterministic), so no separate reduction pass
+// re-reads Ad or s.  Rows-per-lane and the persistent grid size are picked
+// by timing candidates on the batch itself (autotune, cached per batch
+// shape).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "par_body.hpp"
+#include "plan.hpp"
+#include "solvers.hpp"
+
+namespace ccpi {
+
+struct BatchProb {
+  ParArgs g;
+  float *r, *x, *d, *s, *Ad;
+  double *norms;       // [iters]
+  long long nvox, npix;
+  int fp_nbx, fp_nby, bp_nbx, bp_nby;
+  double gamma;
+  float alpha, beta;
+  int it;
+  int pad;
+};
+
+struct BatchLists {
+  int count;
+  const int *fp_off;   // [count + 1] item prefix per kernel type
+  const int *bp_off;
+  const int *vx_off;   // volume chunks
+  const int *px_off;   // sinogram chunks
+  double *part_fp;     // [fp items]
+  double *part_bp;     // [bp items]
+};
+
+constexpr int kChunk = 2048;  // elements per vector work item
+
+__device__ inline int find_prob(const int *off, int count, int item) {
+  int lo = 0, hi = count;  // last p with off[p] <= item
+  while (hi - lo > 1) {
+    const int m = (lo + hi) >> 1;
+    if (off[m] <= item) lo = m; else hi = m;
+  }
+  return lo;
+}
+
+__device__ inline double block_sum_d(double v) {
+  __shared__ double sh[4];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double t = sh[0] + sh[1] + sh[2] + sh[3];
+  __syncthreads();
+  return t;
+}
+
+// FP of d into Ad for every problem (mode 0), or BP of r into s (mode 1),
+// or BP of r into d (mode 2, initial direction).
+template <int RC, int MODE>
+__global__ __launch_bounds__(256) void batch_proj_kernel(BatchProb *__restrict__ P,
+                                                         BatchLists L) {
+  const int *off = MODE == 0 ? L.fp_off : L.bp_off;
+  double *part = MODE == 0 ? L.part_fp : L.part_bp;
+  const int total = off[L.count];
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int p = find_prob(off, L.count, item);
+    const BatchProb &B = P[p];
+    const int local = item - off[p];
+    double ss;
+    if (MODE == 0) {
+      ss = par_fp_thread<RC>(B.g, local % B.fp_nbx, local / B.fp_nbx, threadIdx.x, B.d,
+                             B.Ad, false);
+    } else {
+      ss = par_bp_thread<RC>(B.g, local % B.bp_nbx, local / B.bp_nbx, threadIdx.x, B.r,
+                             MODE == 1 ? B.s : B.d, false);
+    }
+    ss = block_sum_d(ss);
+    if (threadIdx.x == 0) part[item] = ss;
+  }
+}
+
+// Per-problem scalars.  STEP 0: gamma = ||d||^2 (start), 1: alpha,
+// 2: beta / gamma update / norm record.
+template <int STEP>
+__global__ __launch_bounds__(256) void batch_scalar_kernel(BatchProb *__restrict__ P,
+                                                           BatchLists L) {
+  const int p = blockIdx.x;
+  const int *off = STEP == 1 ? L.fp_off : L.bp_off;
+  const double *part = STEP == 1 ? L.part_fp : L.part_bp;
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int q = off[p] + threadIdx.x; q < off[p + 1]; q += 256) s += part[q];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    BatchProb &B = P[p];
+    const double t = sh[0];
+    if (STEP == 0) {
+      B.gamma = t;
+      B.it = 0;
+    } else if (STEP == 1) {
+      B.alpha = (float)(B.gamma / t);
+    } else {
+      B.beta = (float)(t / B.gamma);
+      B.gamma = t;
+      B.norms[B.it] = sqrt(t);
+      B.it = B.it + 1;
+    }
+  }
+}
+
+// x += alpha d, r -= alpha Ad (MODE 0); d = s + beta d (MODE 1)
+template <int MODE>
+__global__ __launch_bounds__(256) void batch_vec_kernel(BatchProb *__restrict__ P,
+                                                        BatchLists L) {
+  const int tv = L.vx_off[L.count];
+  const int total = MODE == 0 ? tv + L.px_off[L.count] : tv;
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const bool vox = item < tv;
+    const int it2 = vox ? item : item - tv;
+    const int *off = vox ? L.vx_off : L.px_off;
+    const int p = find_prob(off, L.count, it2);
+    const BatchProb &B = P[p];
+    const long long n = vox ? B.nvox : B.npix;
+    const long long q0 = (long long)(it2 - off[p]) * kChunk;
+    const long long q1 = q0 + kChunk < n ? q0 + kChunk : n;
+    if (MODE == 0) {
+      const float a = B.alpha;
+      if (vox)
+        for (long long q = q0 + threadIdx.x; q < q1; q += 256) B.x[q] = B.x[q] + a * B.d[q];
+      else
+        for (long long q = q0 + threadIdx.x; q < q1; q += 256) B.r[q] = B.r[q] + (-a) * B.Ad[q];
+    } else {
+      const float b = B.beta;
+      for (long long q = q0 + threadIdx.x; q < q1; q += 256) B.d[q] = b * B.d[q] + B.s[q];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct Tuned {
+  int rc;
+  int grid;
+};
+
+std::mutex g_tune_mu;
+std::map<std::string, Tuned> g_tuned;
+
+template <int RC>
+void launch_proj(int mode, int grid, BatchProb *P, const BatchLists &L, hipStream_t st) {
+  if (mode == 0)
+    hipLaunchKernelGGL((batch_proj_kernel<RC, 0>), dim3(grid), dim3(256), 0, st, P, L);
+  else if (mode == 1)
+    hipLaunchKernelGGL((batch_proj_kernel<RC, 1>), dim3(grid), dim3(256), 0, st, P, L);
+  else
+    hipLaunchKernelGGL((batch_proj_kernel<RC, 2>), dim3(grid), dim3(256), 0, st, P, L);
+}
+
+void proj(int rc, int mode, int grid, BatchProb *P, const BatchLists &L, hipStream_t st) {
+  if (rc == 8)
+    launch_proj<8>(mode, grid, P, L, st);
+  else if (rc == 32)
+    launch_proj<32>(mode, grid, P, L, st);
+  else
+    launch_proj<16>(mode, grid, P, L, st);
+  CCPI_HIP(hipGetLastError());
+}
+
+struct BatchRun {
+  std::vector<std::shared_ptr<ParPlan>> plans;
+  std::vector<BatchProb> host;
+  DeviceBuffer probs, offs, part_fp, part_bp, work, norms;
+  BatchLists L{};
+  int count = 0;
+  int fp_total = 0, bp_total = 0;
+};
+
+// Item lists for row/slice blocks of `rc`.
+void layout(BatchRun &R, int rc, hipStream_t st) {
+  std::vector<int> off(4 * (R.count + 1), 0);
+  for (int p = 0; p < R.count; p++) {
+    BatchProb &B = R.host[p];
+    const ParArgs &g = B.g;
+    B.fp_nbx = (g.na * g.nh + 255) / 256;
+    B.fp_nby = (g.nv + rc - 1) / rc;
+    B.bp_nbx = (g.nx * g.ny + 255) / 256;
+    B.bp_nby = (g.nz + rc - 1) / rc;
+    off[p + 1] = off[p] + B.fp_nbx * B.fp_nby;
+    off[(R.count + 1) + p + 1] = off[(R.count + 1) + p] + B.bp_nbx * B.bp_nby;
+    off[2 * (R.count + 1) + p + 1] =
+        off[2 * (R.count + 1) + p] + (int)((B.nvox + kChunk - 1) / kChunk);
+    off[3 * (R.count + 1) + p + 1] =
+        off[3 * (R.count + 1) + p] + (int)((B.npix + kChunk - 1) / kChunk);
+  }
+  R.fp_total = off[R.count];
+  R.bp_total = off[2 * R.count + 1];
+  R.offs.upload(off, st);
+  R.part_fp.alloc((size_t)std::max(R.fp_total, 1) * sizeof(double));
+  R.part_bp.alloc((size_t)std::max(R.bp_total, 1) * sizeof(double));
+  int *o = R.offs.as<int>();
+  R.L.count = R.count;
+  R.L.fp_off = o;
+  R.L.bp_off = o + (R.count + 1);
+  R.L.vx_off = o + 2 * (R.count + 1);
+  R.L.px_off = o + 3 * (R.count + 1);
+  R.L.part_fp = R.part_fp.as<double>();
+  R.L.part_bp = R.part_bp.as<double>();
+  R.probs.upload(R.host, st);
+}
+
+int cu_count() {
+  int dev = 0, n = 0;
+  CCPI_HIP(hipGetDevice(&dev));
+  CCPI_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+  return std::max(n, 1);
+}
+
+}  // namespace
+
+void run_pb_cgls_batched(const std::vector<Geometry> &geoms, float *const *b,
+                         float *const *x, int iters, double *norms_out, hipStream_t caller) {
+  BatchRun R;
+  R.count = (int)geoms.size();
+  if (R.count == 0) return;
+  CCPI_HIP(hipStreamSynchronize(caller));
+  hipStream_t st;
+  CCPI_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } guard{st};
+
+  // per-problem work vectors carved from one allocation
+  size_t total = 0;
+  for (const Geometry &G : geoms) total += 2 * G.n_vox() + G.n_pix();
+  R.work.alloc(total * sizeof(float));
+  R.norms.alloc((size_t)R.count * std::max(iters, 1) * sizeof(double));
+  float *w = R.work.as<float>();
+  std::string sig;
+  for (int p = 0; p < R.count; p++) {
+    const Geometry &G = geoms[p];
+    CCPI_REQUIRE(G.kind == CCPI_PARALLEL, CCPI_E_ARG, "batched CGLS takes parallel-beam problems");
+    R.plans.push_back(par_plan(G, st));
+    BatchProb B{};
+    B.g = R.plans.back()->args;
+    B.r = b[p];
+    B.x = x[p];
+    B.d = w;
+    w += G.n_vox();
+    B.s = w;
+    w += G.n_vox();
+    B.Ad = w;
+    w += G.n_pix();
+    B.norms = R.norms.as<double>() + (size_t)p * std::max(iters, 1);
+    B.nvox = (long long)G.n_vox();
+    B.npix = (long long)G.n_pix();
+    R.host.push_back(B);
+    sig += std::to_string(G.na) + "," + std::to_string(G.nh) + "," + std::to_string(G.nv) +
+           "," + std::to_string(G.nx) + "," + std::to_string(G.ny) + "," + std::to_string(G.nz) + ";";
+  }
+
+  // autotune rows-per-lane and the persistent grid once per batch shape
+  Tuned T{16, 4 * cu_count()};
+  bool have = false;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tuned.find(sig);
+    if (it != g_tuned.end()) {
+      T = it->second;
+      have = true;
+    }
+  }
+  if (!have) {
+    hipEvent_t e0, e1;
+    CCPI_HIP(hipEventCreate(&e0));
+    CCPI_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    for (int rc : {8, 16, 32}) {
+      layout(R, rc, st);
+      for (int mult : {2, 4, 8}) {
+        const int grid = mult * cu_count();
+        // FP of d (scratch, contents irrelevant) and BP of r into s: the
+        // two projector launches of an iteration, outputs discarded
+        proj(rc, 0, grid, R.probs.as<BatchProb>(), R.L, st);
+        CCPI_HIP(hipEventRecord(e0, st));
+        proj(rc, 0, grid, R.probs.as<BatchProb>(), R.L, st);
+        proj(rc, 1, grid, R.probs.as<BatchProb>(), R.L, st);
+        CCPI_HIP(hipEventRecord(e1, st));
+        CCPI_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        CCPI_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          T = Tuned{rc, grid};
+        }
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tuned[sig] = T;
+  }
+  layout(R, T.rc, st);
+  BatchProb *P = R.probs.as<BatchProb>();
+  const int vgrid = 4 * cu_count();
+
+  // start: d = A^T b, gamma = ||d||^2 (cgls.cpp:45-49)
+  proj(T.rc, 2, T.grid, P, R.L, st);
+  hipLaunchKernelGGL(batch_scalar_kernel<0>, dim3(R.count), dim3(256), 0, st, P, R.L);
+  CCPI_HIP(hipGetLastError());
+
+  // one iteration, captured once and replayed
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  CCPI_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  proj(T.rc, 0, T.grid, P, R.L, st);
+  hipLaunchKernelGGL(batch_scalar_kernel<1>, dim3(R.count), dim3(256), 0, st, P, R.L);
+  hipLaunchKernelGGL(batch_vec_kernel<0>, dim3(vgrid), dim3(256), 0, st, P, R.L);
+  proj(T.rc, 1, T.grid, P, R.L, st);
+  hipLaunchKernelGGL(batch_scalar_kernel<2>, dim3(R.count), dim3(256), 0, st, P, R.L);
+  hipLaunchKernelGGL(batch_vec_kernel<1>, dim3(vgrid), dim3(256), 0, st, P, R.L);
+  CCPI_HIP(hipStreamEndCapture(st, &graph));
+  struct GraphGuard {
+    hipGraph_t g;
+    hipGraphExec_t e;
+    ~GraphGuard() {
+      if (e) (void)hipGraphExecDestroy(e);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } gg{graph, nullptr};
+  CCPI_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  gg.e = exec;
+  for (int it = 0; it < iters; it++) CCPI_HIP(hipGraphLaunch(exec, st));
+  if (norms_out && iters > 0)
+    CCPI_HIP(hipMemcpyAsync(norms_out, R.norms.p, (size_t)R.count * iters * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+  CCPI_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace ccpi

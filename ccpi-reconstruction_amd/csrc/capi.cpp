@@ -241,17 +241,18 @@ int ccpi_projector_nnz(const ccpi_geometry *g, unsigned long long *nnz, void *st
 int ccpi_pb_cgls_batched(const ccpi_geometry *geoms, int count, float *const *b,
                          float *const *x, int iterations, double *norms, int flags,
                          void *stream) {
+  (void)flags;
   return guarded([&] {
     CCPI_REQUIRE(geoms && b && x && count >= 0, CCPI_E_ARG, "bad batch arguments");
+    CCPI_REQUIRE(iterations >= 0, CCPI_E_ARG, "negative iteration count");
+    std::vector<ccpi::Geometry> G;
     for (int i = 0; i < count; i++) {
       CCPI_REQUIRE(geoms[i].kind == CCPI_PARALLEL, CCPI_E_ARG,
                    "batched CGLS takes parallel-beam problems");
-      ccpi::Projector P(ccpi::Geometry::from(&geoms[i]), S(stream));
-      ccpi::Workspace W;
-      ccpi::run_cgls(P, b[i], x[i], iterations, norms ? norms + (size_t)i * iterations : nullptr,
-                     (flags & CCPI_REDUCE_FP32) != 0, W, S(stream));
+      CCPI_REQUIRE(b[i] && x[i], CCPI_E_ARG, "null data pointer");
+      G.push_back(ccpi::Geometry::from(&geoms[i]));
     }
-    CCPI_HIP(hipStreamSynchronize(S(stream)));
+    ccpi::run_pb_cgls_batched(G, b, x, iterations, norms, S(stream));
   });
 }
 

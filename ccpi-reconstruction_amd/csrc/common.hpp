@@ -61,6 +61,8 @@ struct ConeArgs {
   int na, nh, nv, nx, ny, nz;
   int mid;               // first detector row above the source
   int bp_stride;         // BP row interleave (>= rows per slice)
+  int bp_unique;         // stride guarantees distinct slices per instruction
+  int bp_main_safe;      // source inside the volume's z range
   float pzbz, pzbz1, inv_dz;
   float pzdv, z1, znm;   // BP safe-row bounds (c2D.cpp:993-1000)
   double vx, vy, dx, dy; // volume xy origin and voxel size

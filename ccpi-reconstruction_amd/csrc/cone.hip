@@ -221,6 +221,8 @@ __global__ __launch_bounds__(256) void cone_walk_kernel(
 // ---------------------------------------------------------------------------
 
 constexpr int kFpRows = 256;
+// dword 3 of a raw (untyped, stride 0) buffer resource on gfx9 CDNA
+constexpr int kRsrcWord3 = 0x00020000;
 
 __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
     ConeArgs g, const ConeRay *__restrict__ rays, const ConeEnt *__restrict__ ents,
@@ -255,9 +257,13 @@ __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
       const float mz = a1 < az ? a1 : az;
       const float w0 = mz - a0, w1 = a1 - mz;
       const int k2 = k + dir;
-      const float *__restrict__ col = vol + (size_t)en.col * nz;
-      const float c0 = (k >= 0 && k < nz) ? col[k] : 0.f;
-      const float c1 = (k2 >= 0 && k2 < nz) ? col[k2] : 0.f;
+      // the column as a buffer resource (base in SGPRs): slices outside
+      // [0, nz) read 0 through the hardware range check
+      const uint32_t colid = __builtin_amdgcn_readfirstlane(en.col);
+      const __amdgpu_buffer_rsrc_t col = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)(vol + (size_t)colid * nz), (short)0, nz * 4, kRsrcWord3);
+      const float c0 = __builtin_amdgcn_raw_buffer_load_b32(col, k * 4, 0, 0);
+      const float c1 = __builtin_amdgcn_raw_buffer_load_b32(col, k2 * 4, 0, 0);
       acc = acc + (c0 * w0 + c1 * w1);
       a0 = a1;
     }
@@ -376,6 +382,15 @@ __device__ inline int wave_excl_scan(int x, int lane, int &total) {
   return incl - x;
 }
 
+// One LDS entry of a column's ray list.
+struct BpEnt {
+  uint32_t ray;   // a * nh + h
+  float a0, a1;   // alpha interval inside the column
+  float ai;       // a0 / voxel z size
+};
+
+__device__ inline int lds_pad4(int n) { return (n + 3) & ~3; }
+
 __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
     ConeArgs g, const float *__restrict__ dpix, float *__restrict__ vol,
     int *__restrict__ err) {
@@ -387,10 +402,8 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int colid = blockIdx.x * kBpWaves + wave;
   if (colid >= g.nx * g.ny) return;
-  float *acc = smem + (nz + 1) + wave * (nz + 3 * kBpCap);
-  uint32_t *e_ray = (uint32_t *)(acc + nz);
-  float *e_a0 = acc + nz + kBpCap;
-  float *e_a1 = acc + nz + 2 * kBpCap;
+  float *acc = smem + lds_pad4(nz + 1) + wave * (lds_pad4(nz) + 4 * kBpCap);
+  BpEnt *ent = (BpEnt *)(acc + lds_pad4(nz));
   const int i = colid / g.ny, j = colid - (colid / g.ny) * g.ny;
   const double x0 = g.vx + (double)i * g.dx, xn = g.vx + (double)(i + 1) * g.dx;
   const double y0 = g.vy + (double)j * g.dy, yn = g.vy + (double)(j + 1) * g.dy;
@@ -422,6 +435,7 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
 
   for (int k = lane; k < nz; k += 64) acc[k] = vcol[k];
 
+  const int S = g.bp_stride;
   // pass 2: batches of angles -> LDS entry list -> row sweep
   for (int base = 0; base < g.na; base += 64) {
     const int a = base + lane;
@@ -451,9 +465,7 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
         for (int h = hmin; h <= hmax; h++) {
           float a0, a1;
           if (cone_col_alpha(g, t, h, x0, xn, y0, yn, a0, a1)) {
-            e_ray[q] = (uint32_t)(a * g.nh + h);
-            e_a0[q] = a0;
-            e_a1[q] = a1;
+            ent[q] = BpEnt{(uint32_t)(a * g.nh + h), a0, a1, a0 * g.inv_dz};
             ++q;
           }
         }
@@ -461,22 +473,41 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
       }
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      // sweep the batch: row block outer, entries inner, so each lane's row
-      // constants are loaded once per block.  Lane L takes rows
-      // vb + S*L + t (t < S): with S at least the largest number of rows
-      // per slice, the lanes of one instruction hit distinct slices and the
-      // column update is a plain LDS read-add-write.  Slice indices are
-      // non-decreasing across lanes, so a duplicate can only sit in the
-      // next lane; one DPP shift detects it and that instruction falls back
-      // to ds_add.
-      const int S = g.bp_stride;
-      const int npass = one_set ? 1 : 4;
+      // Sweep the batch: row block outer, entries inner.  Lane L takes rows
+      // vb + S*L + t (t < S).  S bounds the rows per slice, so the lanes of
+      // one instruction hit distinct slices and the column update is a
+      // plain LDS read-add-write (bp_unique); otherwise ds_add.
       for (int vb = 0; vb < nv; vb += 64 * S) {
-        for (int t = 0; t < S; ++t) {
-          const int v = vb + S * lane + t;
+        for (int tt = 0; tt < S; ++tt) {
+          const int v = vb + S * lane + tt;
+          const int vfirst = vb + tt, vlast = vb + tt + 63 * S;
           const bool live = v < nv;
           const int vv = live ? v : nv - 1;
           const float dzv = g.dz[vv], izv = g.iz[vv];
+          const bool lower = v < mid;
+          const int kzoff = lower ? 0 : 1, dir = lower ? -1 : 1;
+          if (g.bp_main_safe && g.bp_unique && one_set && vfirst >= min_v && vlast < max_v) {
+            // every lane a main row inside the volume (min_v / max_v bound
+            // the slices, c2D.cpp:657-676): no range checks
+            for (int e = 0; e < used; ++e) {
+              const BpEnt E = ent[e];
+              const uint32_t ray = __builtin_amdgcn_readfirstlane(E.ray);
+              const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
+                  (void *)(dpix + (size_t)ray * nv), (short)0, nv * 4, kRsrcWord3);
+              const float p = __builtin_amdgcn_raw_buffer_load_b32(prow, v * 4, 0, 0);
+              // clamps are insurance only: main rows stay in [0, nz)
+              const int k = min(max((int)(g.pzbz + E.ai * dzv), 0), nz - 1);
+              const int k2 = min(max(k + dir, 0), nz - 1);
+              const float az = s_vz[k + kzoff] * izv;
+              const float mz = E.a1 < az ? E.a1 : az;
+              acc[k] = acc[k] + p * (mz - E.a0);
+              __builtin_amdgcn_wave_barrier();
+              acc[k2] = acc[k2] + p * (E.a1 - mz);
+              __builtin_amdgcn_wave_barrier();
+            }
+            continue;
+          }
+          const int npass = one_set ? 1 : 4;
           for (int pass = 0; pass < npass; ++pass) {
             // row set: 0 lower edge, 1 lower main, 2 upper main, 3 upper edge
             int set;
@@ -492,38 +523,32 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
                                            : (v >= max_v));
             }
             if (!__any(member)) continue;
-            const bool lower = set <= 1;
+            const bool low = set <= 1;
             const float kbase = set == 0 ? g.pzbz1 : g.pzbz;
             const int ksub = set == 0 ? 1 : 0;
-            const int kzoff = lower ? 0 : 1;
-            const int dir = lower ? -1 : 1;
-            const float *__restrict__ pcol = dpix + v;
+            const int zo = low ? 0 : 1, dr = low ? -1 : 1;
             for (int e = 0; e < used; ++e) {
-              const uint32_t ray = e_ray[e];
-              const float a0 = e_a0[e], a1 = e_a1[e];
-              const float ai = a0 * g.inv_dz;
-              const int k = (int)(kbase + ai * dzv) - ksub;
-              const int k2 = k + dir;
+              const BpEnt E = ent[e];
+              const uint32_t ray = __builtin_amdgcn_readfirstlane(E.ray);
+              const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
+                  (void *)(dpix + (size_t)ray * nv), (short)0, nv * 4, kRsrcWord3);
+              const float p = __builtin_amdgcn_raw_buffer_load_b32(prow, vv * 4, 0, 0);
+              const int k = (int)(kbase + E.ai * dzv) - ksub;
+              const int k2 = k + dr;
               const bool ok0 = member && k >= 0 && k < nz;
               const bool ok1 = member && k2 >= 0 && k2 < nz;
-              float p = 0.f;
-              if (member) p = pcol[(size_t)ray * nv];
-              const int kz = min(max(k + kzoff, 0), nz);
+              const int kz = min(max(k + zo, 0), nz);
               const float az = s_vz[kz] * izv;
-              const float mz = a1 < az ? a1 : az;
-              const float c0 = p * (mz - a0), c1 = p * (a1 - mz);
-              // unique negative keys for idle lanes never compare equal
-              const int key0 = ok0 ? k : -1 - lane, key1 = ok1 ? k2 : -1 - lane;
-              const int prev0 = __builtin_amdgcn_update_dpp(0x7fffffff, key0, 0x138, 0xf, 0xf, false);
-              const int prev1 = __builtin_amdgcn_update_dpp(0x7fffffff, key1, 0x138, 0xf, 0xf, false);
-              if (__any(key0 == prev0 || key1 == prev1)) {
-                if (ok0) atomicAdd(&acc[k], c0);
-                if (ok1) atomicAdd(&acc[k2], c1);
-              } else {
+              const float mz = E.a1 < az ? E.a1 : az;
+              const float c0 = p * (mz - E.a0), c1 = p * (E.a1 - mz);
+              if (g.bp_unique) {
                 if (ok0) acc[k] = acc[k] + c0;
                 __builtin_amdgcn_wave_barrier();
                 if (ok1) acc[k2] = acc[k2] + c1;
                 __builtin_amdgcn_wave_barrier();
+              } else {
+                if (ok0) atomicAdd(&acc[k], c0);
+                if (ok1) atomicAdd(&acc[k2], c1);
               }
             }
           }
@@ -648,7 +673,19 @@ void cone_build_plan(ConePlan &P, hipStream_t st) {
     double pitch = 1e300;
     for (int v = 1; v < nv; v++) pitch = std::min(pitch, G.v_pix[v] - G.v_pix[v - 1]);
     const double rows_per_slice = G.vsize[2] / (amin * pitch);
-    A.bp_stride = std::max(1, std::min(16, (int)std::ceil(rows_per_slice * 1.05)));
+    const int want = std::max(1, (int)std::ceil(rows_per_slice * 1.05));
+    A.bp_stride = std::min(16, want);
+    // lanes S rows apart are >= 1.05 slices apart: distinct slices
+    A.bp_unique = want <= 16 ? 1 : 0;
+    // a source inside the volume's z range keeps main rows' slices in range
+    // and uniformly spaced rows make min_v / max_v exact row bounds
+    // (they are computed from the first pixel step, c2D.cpp:995-1000)
+    bool uniform = true;
+    const double st0 = G.v_pix[1] - G.v_pix[0];
+    for (int v = 1; v < nv; v++)
+      uniform = uniform && std::fabs((G.v_pix[v] - G.v_pix[v - 1]) - st0) <= 1e-9 * std::fabs(st0);
+    A.bp_main_safe =
+        (uniform && sz >= G.origin[2] && sz < G.origin[2] + (double)nz * G.vsize[2]) ? 1 : 0;
   }
   P.atab.upload(atab, st);
   P.hpix.upload(G.h_pix, st);
@@ -715,7 +752,8 @@ void cone_nnz(const ConePlan &P, unsigned long long *count, hipStream_t st) {
 }
 
 size_t cone_bp_lds_bytes(int nz) {
-  return sizeof(float) * ((size_t)(nz + 1) + (size_t)kBpWaves * (nz + 3 * kBpCap));
+  const size_t pad1 = (size_t)((nz + 1 + 3) & ~3), pad0 = (size_t)((nz + 3) & ~3);
+  return sizeof(float) * (pad1 + (size_t)kBpWaves * (pad0 + 4 * kBpCap));
 }
 
 void cone_scale(const ConePlan &P, const float *pix, float *dpix, int *err,

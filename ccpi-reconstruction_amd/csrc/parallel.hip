@@ -25,6 +25,7 @@
 
 #include "common.hpp"
 #include "plan.hpp"
+#include "par_body.hpp"
 
 namespace ccpi {
 
@@ -33,222 +34,13 @@ constexpr int kRows = 16;
 __global__ __launch_bounds__(256) void par_fp_kernel(ParArgs g,
                                                      const float *__restrict__ vol,
                                                      float *__restrict__ sino) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= g.na * g.nh) return;
-  const int a = r / g.nh, h = r - a * g.nh;
-  const int v0 = blockIdx.y * kRows;
-  const int nz = g.nz, nx = g.nx, ny = g.ny;
-  float *__restrict__ out = sino + (size_t)r * g.nv + v0;
-  const int nrow = min(kRows, g.nv - v0);
-  float acc[kRows];
-  int kmap[kRows];
-#pragma unroll
-  for (int q = 0; q < kRows; q++) {
-    acc[q] = q < nrow ? out[q] : 0.f;
-    const int k = q < nrow ? g.map[v0 + q] : -1;
-    kmap[q] = (k >= 0 && k < nz) ? k : -1;
-  }
-  const double c = g.atab[5 * a], s = g.atab[5 * a + 1];
-  const double p2x = c * g.D - s * g.h_pix[h];
-  const double p2y = s * g.D + c * g.h_pix[h];
-
-  auto add = [&](int x, int y, float l) {
-    const float *__restrict__ col = vol + ((size_t)x * ny + y) * nz;
-#pragma unroll
-    for (int q = 0; q < kRows; q++)
-      if (kmap[q] >= 0) acc[q] = acc[q] + col[kmap[q]] * l;
-  };
-
-  if (fabs(c) < kEps) {
-    if (fabs(s) >= kEps) {  // line parallel to y (p2D.cpp:233-260)
-      const int i = (int)floor((p2x - g.ox) / g.dx);
-      if (i >= 0 && i < nx) {
-        const float l = (float)g.dy;
-        if (s < 0.0)
-          for (int j = ny - 1; j >= 0; j--) add(i, j, l);
-        else
-          for (int j = 0; j < ny; j++) add(i, j, l);
-      }
-    }
-  } else if (fabs(s) < kEps) {  // line parallel to x (p2D.cpp:261-283)
-    const int j = (int)floor((p2y - g.oy) / g.dy);
-    if (j >= 0 && j < ny) {
-      const float l = (float)g.dx;
-      if (c < 0.0)
-        for (int i = nx - 1; i >= 0; i--) add(i, j, l);
-      else
-        for (int i = 0; i < nx; i++) add(i, j, l);
-    }
-  } else {  // general line (p2D.cpp:284-458)
-    const double ix = 1.0 / c, iy = 1.0 / s;
-    const double ax0 = (g.ox - p2x) * ix, ay0 = (g.oy - p2y) * iy;
-    const double axn = (g.ox + (double)nx * g.dx - p2x) * ix;
-    const double ayn = (g.oy + (double)ny * g.dy - p2y) * iy;
-    const double amin = fmax(fmax(fmin(ax0, axn), fmin(ay0, ayn)), -g.dconv);
-    const double amax = fmin(fmin(fmax(ax0, axn), fmax(ay0, ayn)), 0.0);
-    if (amin < amax - kEps) {
-      const int sx = c > 0.0 ? 1 : -1, sy = s > 0.0 ? 1 : -1;
-      int x, y;
-      {
-        const double xface = sx > 0 ? ax0 : axn, yface = sy > 0 ? ay0 : ayn;
-        const int xin = sx > 0 ? 0 : nx - 1, yin = sy > 0 ? 0 : ny - 1;
-        bool on_x = amin == xface, on_y = amin == yface;
-        if (sx > 0 && sy < 0) {
-          if (on_y) on_x = false;
-        } else if (!(sx < 0 && sy < 0)) {
-          if (on_x) on_y = false;
-        }
-        if (on_x) {
-          x = xin;
-          y = on_y ? yin : (int)floor((p2y + amin * s - g.oy) / g.dy);
-        } else if (on_y) {
-          y = yin;
-          x = (int)floor((p2x + amin * c - g.ox) / g.dx);
-        } else {
-          x = 0;
-          y = 0;
-        }
-        x = min(max(x, 0), nx - 1);
-        y = min(max(y, 0), ny - 1);
-      }
-      const int bxs = sx > 0, bys = sy > 0;
-      double ap = amin;
-      while (x >= 0 && x < nx && y >= 0 && y < ny) {
-        const double nxa = (g.ox + (double)(x + bxs) * g.dx - p2x) * ix;
-        const double nya = (g.oy + (double)(y + bys) * g.dy - p2y) * iy;
-        const int cx = x, cy = y;
-        double nap;
-        if (nxa < nya - kEps) {
-          nap = nxa;
-          x += sx;
-        } else if (nya < nxa - kEps) {
-          nap = nya;
-          y += sy;
-        } else {
-          nap = fmax(nxa, nya);
-          x += sx;
-          y += sy;
-        }
-        add(cx, cy, (float)(nap - ap));
-        ap = nap;
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < kRows; q++)
-    if (q < nrow) out[q] = acc[q];
+  par_fp_thread<kRows>(g, blockIdx.x, blockIdx.y, threadIdx.x, vol, sino, true);
 }
 
 __global__ __launch_bounds__(256) void par_bp_kernel(ParArgs g,
                                                      const float *__restrict__ sino,
                                                      float *__restrict__ vol) {
-  const int colid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (colid >= g.nx * g.ny) return;
-  const int i = colid / g.ny, j = colid - (colid / g.ny) * g.ny;
-  const int k0 = blockIdx.y * kRows;
-  const int nz = g.nz, nv = g.nv, nh = g.nh;
-  const int nk = min(kRows, nz - k0);
-  float *__restrict__ vcol = vol + (size_t)colid * nz + k0;
-  float acc[kRows];
-#pragma unroll
-  for (int q = 0; q < kRows; q++) acc[q] = q < nk ? vcol[q] : 0.f;
-  const double x0 = g.ox + (double)i * g.dx, xn = g.ox + (double)(i + 1) * g.dx;
-  const double y0 = g.oy + (double)j * g.dy, yn = g.oy + (double)(j + 1) * g.dy;
-  const int T = g.map_type;
-
-  for (int a = 0; a < g.na; a++) {
-    const double *t = g.atab + 5 * a;
-    const double c = t[0], s = t[1];
-    int lo, hi, kind;
-    double ymin = 0, ymax = 0, lim0 = 0, lim1 = 0;
-    if (fabs(c) < kEps) {  // p2D.cpp:808-844
-      kind = s < 0.0 ? 1 : 2;
-      lim0 = x0;
-      lim1 = xn;
-      if (s < 0.0) {
-        lo = (int)floor(x0 * g.ihp - g.hp0);
-        hi = (int)floor((xn - kEps) * g.ihp - g.hp0);
-      } else {
-        lo = (int)ceil((-xn + kEps) * g.ihp - g.hp0);
-        hi = (int)ceil(-x0 * g.ihp - g.hp0);
-      }
-      lo = max(lo, 0);
-      hi = min(hi, nh - 1);
-    } else if (fabs(s) < kEps) {  // p2D.cpp:845-880
-      kind = c < 0.0 ? 2 : 1;
-      lim0 = y0;
-      lim1 = yn;
-      if (c < 0.0) {
-        lo = (int)ceil((-yn + kEps) * g.ihp - g.hp0);
-        hi = (int)ceil(-y0 * g.ihp - g.hp0);
-      } else {
-        lo = (int)floor(y0 * g.ihp - g.hp0);
-        hi = (int)floor((yn - kEps) * g.ihp - g.hp0);
-      }
-      lo = max(lo, 0);
-      hi = min(hi, nh - 1);
-    } else {  // p2D.cpp:881-915
-      kind = 0;
-      if (c > 0.0) {
-        if (s > 0.0) { ymin = y0 * c - xn * s; ymax = yn * c - x0 * s; }
-        else         { ymin = y0 * c - x0 * s; ymax = yn * c - xn * s; }
-      } else {
-        if (s > 0.0) { ymin = yn * c - xn * s; ymax = y0 * c - x0 * s; }
-        else         { ymin = yn * c - x0 * s; ymax = y0 * c - xn * s; }
-      }
-      // clamp at 0: the floor can round to -1 when ymin ~ h_pix[0]
-      lo = ymin < g.h_pix[0] ? 0 : max((int)floor(ymin * g.ihp - g.hp0), 0);
-      hi = min((int)floor(ymax * g.ihp - g.hp0), nh - 1);
-    }
-    for (int h = lo; h <= hi; h++) {
-      const double hp = g.h_pix[h];
-      double w;
-      if (kind != 0) {
-        const double tt = kind == 1 ? hp : -hp;
-        if (!(tt >= lim0 && tt < lim1)) continue;
-        w = fabs(c) < kEps ? g.dy : g.dx;
-      } else {  // trapezoid (p2D.cpp:916-938)
-        const double ybot = ymin + t[2], ytop = ymax - t[2], l = t[4];
-        if (!(hp > ymin + kEps)) continue;
-        if (hp < ybot)
-          w = l * (hp - ymin) * t[3];
-        else if (hp <= ytop)
-          w = l;
-        else if (hp < ymax - kEps)
-          w = l * (ymax - hp) * t[3];
-        else
-          break;
-      }
-      const float wf = (float)w;
-      const float *__restrict__ p = sino + ((size_t)a * nh + h) * nv;
-      if (T == 1) {
-#pragma unroll
-        for (int q = 0; q < kRows; q++)
-          if (q < nk && k0 + q < nv) acc[q] = acc[q] + p[k0 + q] * wf;
-      } else if (T == 2 || T == 4) {  // rows summed before scaling (p2D.cpp:694-761)
-#pragma unroll
-        for (int q = 0; q < kRows; q++) {
-          const int k = k0 + q;
-          if (q < nk && (k + 1) * T <= nv) {
-            float sum = p[k * T];
-            for (int u = 1; u < T; u++) sum = sum + p[k * T + u];
-            acc[q] = acc[q] + sum * wf;
-          }
-        }
-      } else {
-#pragma unroll
-        for (int q = 0; q < kRows; q++) {
-          if (q < nk) {
-            const int k = k0 + q;
-            for (int v = g.rlo[k]; v < g.rhi[k]; v++) acc[q] = acc[q] + p[v] * wf;
-          }
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < kRows; q++)
-    if (q < nk) vcol[q] = acc[q];
+  par_bp_thread<kRows>(g, blockIdx.x, blockIdx.y, threadIdx.x, sino, vol, true);
 }
 
 // Nonzero system-matrix entries: per ray, crossed cells with nonzero xy

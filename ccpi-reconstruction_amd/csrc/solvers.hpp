@@ -58,5 +58,9 @@ void run_sirt(const Projector &P, const float *b, float *x, int iters,
 void run_mlem(const Projector &P, const float *b, float *x, int iters,
               Workspace &W, hipStream_t st);
 unsigned long long projector_nnz(const Projector &P, hipStream_t st);
+// batched.hip: parallel-beam problems of uneven sizes, one persistent launch
+// per step, iteration replayed from a hipGraph (FP64 reductions)
+void run_pb_cgls_batched(const std::vector<Geometry> &geoms, float *const *b,
+                         float *const *x, int iters, double *norms_out, hipStream_t st);
 
 }  // namespace ccpi
