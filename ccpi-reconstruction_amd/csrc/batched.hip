@@ -23,6 +23,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -128,9 +130,9 @@ __global__ __launch_bounds__(256) void batch_scalar_kernel(BatchProb *__restrict
       B.gamma = t;
       B.it = 0;
     } else if (STEP == 1) {
-      B.alpha = (float)(B.gamma / t);
+      B.alpha = t > 0.0 ? (float)(B.gamma / t) : 0.f;  // zero direction: stop
     } else {
-      B.beta = (float)(t / B.gamma);
+      B.beta = B.gamma > 0.0 ? (float)(t / B.gamma) : 0.f;
       B.gamma = t;
       B.norms[B.it] = sqrt(t);
       B.it = B.it + 1;
@@ -294,7 +296,16 @@ void run_pb_cgls_batched(const std::vector<Geometry> &geoms, float *const *b,
   // autotune rows-per-lane and the persistent grid once per batch shape
   Tuned T{16, 4 * cu_count()};
   bool have = false;
-  {
+  if (const char *env = std::getenv("CCPI_BATCH_TUNE")) {
+    // "rc,blocks_per_cu" pins the launch shape (skips the timing sweep)
+    int rc = 16, mult = 4;
+    if (std::sscanf(env, "%d,%d", &rc, &mult) == 2 && (rc == 8 || rc == 16 || rc == 32) &&
+        mult > 0) {
+      T = Tuned{rc, mult * cu_count()};
+      have = true;
+    }
+  }
+  if (!have) {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = g_tuned.find(sig);
     if (it != g_tuned.end()) {

@@ -63,6 +63,9 @@ struct ConeArgs {
   int bp_stride;         // BP row interleave (>= rows per slice)
   int bp_unique;         // stride guarantees distinct slices per instruction
   int bp_main_safe;      // source inside the volume's z range
+  float zb, zt;          // volume bottom / top relative to source z
+  const float *alim;     // [nv] walk parameter where row v leaves the volume's
+                         // z range (>= 2 where it never does)
   float pzbz, pzbz1, inv_dz;
   float pzdv, z1, znm;   // BP safe-row bounds (c2D.cpp:993-1000)
   double vx, vy, dx, dy; // volume xy origin and voxel size

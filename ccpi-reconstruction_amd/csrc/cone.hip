@@ -224,6 +224,12 @@ constexpr int kFpRows = 256;
 // dword 3 of a raw (untyped, stride 0) buffer resource on gfx9 CDNA
 constexpr int kRsrcWord3 = 0x00020000;
 
+// 32-bit raw buffer load reinterpreted as float (the builtin returns the
+// bits as an unsigned int); an offset outside the resource reads 0.
+__device__ inline float buf_load_f32(__amdgpu_buffer_rsrc_t r, int byte_off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+}
+
 __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
     ConeArgs g, const ConeRay *__restrict__ rays, const ConeEnt *__restrict__ ents,
     const float *__restrict__ vol, float *__restrict__ sino) {
@@ -231,12 +237,21 @@ __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
   for (int q = threadIdx.x; q <= g.nz; q += kFpRows) s_vz[q] = g.vz[q];
   __syncthreads();
   const int r = blockIdx.x;
-  const int v = blockIdx.y * kFpRows + threadIdx.x;
-  if (v >= g.nv) return;
+  const int v0 = blockIdx.y * kFpRows + threadIdx.x;
+  // lanes past the last row mirror row nv - 1 and store nothing, so the
+  // wave-wide reductions below see every lane
+  const bool live = v0 < g.nv;
+  const int v = live ? v0 : g.nv - 1;
   const int h = r % g.nh;
   const ConeRay R = rays[r];
   float *out = sino + (size_t)r * g.nv + v;
   float acc = *out;
+  // A row whose ray has left the volume's z range at the start of a
+  // segment adds nothing from there on (c2D.cpp:153-187 break): the wave
+  // stops at the largest such walk parameter of its rows.
+  float amax = g.alim[v];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) amax = fmaxf(amax, __shfl_xor(amax, d, 64));
   if (R.n > 1) {
     const float dzv = g.dz[v], izv = g.iz[v];
     const bool lower = v < g.mid;
@@ -245,6 +260,7 @@ __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
     const ConeEnt *__restrict__ e = ents + R.off;
     float a0 = e[0].alpha;
     for (int m = 1; m < R.n; ++m) {
+      if (a0 >= amax) break;
       const ConeEnt en = e[m];
       const float a1 = en.alpha;
       const float ai = a0 * g.inv_dz;
@@ -262,13 +278,13 @@ __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
       const uint32_t colid = __builtin_amdgcn_readfirstlane(en.col);
       const __amdgpu_buffer_rsrc_t col = __builtin_amdgcn_make_buffer_rsrc(
           (void *)(vol + (size_t)colid * nz), (short)0, nz * 4, kRsrcWord3);
-      const float c0 = __builtin_amdgcn_raw_buffer_load_b32(col, k * 4, 0, 0);
-      const float c1 = __builtin_amdgcn_raw_buffer_load_b32(col, k2 * 4, 0, 0);
+      const float c0 = buf_load_f32(col, k * 4);
+      const float c1 = buf_load_f32(col, k2 * 4);
       acc = acc + (c0 * w0 + c1 * w1);
       a0 = a1;
     }
   }
-  *out = acc * g.dconv[(size_t)h * g.nv + v];
+  if (live) *out = acc * g.dconv[(size_t)h * g.nv + v];
 }
 
 // Counts the nonzero-weight voxel operands of cone_fp_kernel (same row
@@ -387,7 +403,32 @@ struct BpEnt {
   uint32_t ray;   // a * nh + h
   float a0, a1;   // alpha interval inside the column
   float ai;       // a0 / voxel z size
+  int vlo, vhi;   // rows still inside the volume's z range at a0
+  int pad0, pad1;
 };
+
+// Rows whose ray is inside the volume's z range at walk parameter a0:
+// alim is increasing below the source row and decreasing above it.
+__device__ inline void live_rows(const ConeArgs &g, float a0, int &vlo, int &vhi) {
+  if (!g.bp_main_safe) {
+    vlo = 0;
+    vhi = g.nv;
+    return;
+  }
+  int lo = 0, hi = g.mid;  // first lower row with alim > a0
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (g.alim[m] > a0) hi = m; else lo = m + 1;
+  }
+  vlo = lo;
+  lo = g.mid;
+  hi = g.nv;  // first upper row with alim <= a0
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (g.alim[m] <= a0) hi = m; else lo = m + 1;
+  }
+  vhi = lo;
+}
 
 __device__ inline int lds_pad4(int n) { return (n + 3) & ~3; }
 
@@ -402,7 +443,7 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int colid = blockIdx.x * kBpWaves + wave;
   if (colid >= g.nx * g.ny) return;
-  float *acc = smem + lds_pad4(nz + 1) + wave * (lds_pad4(nz) + 4 * kBpCap);
+  float *acc = smem + lds_pad4(nz + 1) + wave * (lds_pad4(nz) + 8 * kBpCap);
   BpEnt *ent = (BpEnt *)(acc + lds_pad4(nz));
   const int i = colid / g.ny, j = colid - (colid / g.ny) * g.ny;
   const double x0 = g.vx + (double)i * g.dx, xn = g.vx + (double)(i + 1) * g.dx;
@@ -465,7 +506,9 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
         for (int h = hmin; h <= hmax; h++) {
           float a0, a1;
           if (cone_col_alpha(g, t, h, x0, xn, y0, yn, a0, a1)) {
-            ent[q] = BpEnt{(uint32_t)(a * g.nh + h), a0, a1, a0 * g.inv_dz};
+            int vlo, vhi;
+            live_rows(g, a0, vlo, vhi);
+            ent[q] = BpEnt{(uint32_t)(a * g.nh + h), a0, a1, a0 * g.inv_dz, vlo, vhi, 0, 0};
             ++q;
           }
         }
@@ -491,10 +534,11 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
             // the slices, c2D.cpp:657-676): no range checks
             for (int e = 0; e < used; ++e) {
               const BpEnt E = ent[e];
+              if (vfirst >= E.vhi || vlast < E.vlo) continue;  // rows all outside
               const uint32_t ray = __builtin_amdgcn_readfirstlane(E.ray);
               const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
                   (void *)(dpix + (size_t)ray * nv), (short)0, nv * 4, kRsrcWord3);
-              const float p = __builtin_amdgcn_raw_buffer_load_b32(prow, v * 4, 0, 0);
+              const float p = buf_load_f32(prow, v * 4);
               // clamps are insurance only: main rows stay in [0, nz)
               const int k = min(max((int)(g.pzbz + E.ai * dzv), 0), nz - 1);
               const int k2 = min(max(k + dir, 0), nz - 1);
@@ -529,10 +573,11 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
             const int zo = low ? 0 : 1, dr = low ? -1 : 1;
             for (int e = 0; e < used; ++e) {
               const BpEnt E = ent[e];
+              if (vfirst >= E.vhi || vlast < E.vlo) continue;  // rows all outside
               const uint32_t ray = __builtin_amdgcn_readfirstlane(E.ray);
               const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
                   (void *)(dpix + (size_t)ray * nv), (short)0, nv * 4, kRsrcWord3);
-              const float p = __builtin_amdgcn_raw_buffer_load_b32(prow, vv * 4, 0, 0);
+              const float p = buf_load_f32(prow, vv * 4);
               const int k = (int)(kbase + E.ai * dzv) - ksub;
               const int k2 = k + dr;
               const bool ok0 = member && k >= 0 && k < nz;
@@ -687,6 +732,26 @@ void cone_build_plan(ConePlan &P, hipStream_t st) {
     A.bp_main_safe =
         (uniform && sz >= G.origin[2] && sz < G.origin[2] + (double)nz * G.vsize[2]) ? 1 : 0;
   }
+  // Walk parameter past which a row's ray is outside the volume's z range
+  // for good: a row below the source is kLimSlices slices under the bottom
+  // face (its floor-form slice index, and the one below, are then < 0), a
+  // row above it kLimSlices slices over the top (both indices >= nz).  The
+  // margin covers the float rounding of the kernels' slice position.
+  // Without a source inside the z range the cut is off (2 > any alpha).
+  A.zb = (float)(G.origin[2] - sz);
+  A.zt = (float)(G.origin[2] + (double)nz * G.vsize[2] - sz);
+  constexpr double kLimSlices = 0.05;
+  std::vector<float> alim(nv, 2.0f);
+  if (A.bp_main_safe)
+    for (int v = 0; v < nv; v++) {
+      const double d = (double)dz[v];
+      if (d == 0.0) continue;
+      const double lim = v < A.mid ? ((double)A.zb - kLimSlices * G.vsize[2]) / d
+                                   : ((double)A.zt + kLimSlices * G.vsize[2]) / d;
+      alim[v] = (float)std::min(2.0, lim);
+    }
+  P.alim.upload(alim, st);
+  A.alim = P.alim.as<float>();
   P.atab.upload(atab, st);
   P.hpix.upload(G.h_pix, st);
   P.hrange.upload(hr, st);
@@ -753,7 +818,7 @@ void cone_nnz(const ConePlan &P, unsigned long long *count, hipStream_t st) {
 
 size_t cone_bp_lds_bytes(int nz) {
   const size_t pad1 = (size_t)((nz + 1 + 3) & ~3), pad0 = (size_t)((nz + 3) & ~3);
-  return sizeof(float) * (pad1 + (size_t)kBpWaves * (pad0 + 4 * kBpCap));
+  return sizeof(float) * (pad1 + (size_t)kBpWaves * (pad0 + 8 * kBpCap));
 }
 
 void cone_scale(const ConePlan &P, const float *pix, float *dpix, int *err,

@@ -77,13 +77,13 @@ struct DeviceBuffer {
 struct ConePlan {
   Geometry g;
   ConeArgs args{};
-  DeviceBuffer atab, hpix, hrange, dz, iz, vz, dconv, offs, rays, ents;
+  DeviceBuffer atab, hpix, hrange, dz, iz, vz, dconv, offs, rays, ents, alim;
   DeviceBuffer dpix;    // scaled sinogram scratch of the BP
   DeviceBuffer err;     // BP overflow flag
   uint64_t n_entries = 0;
   size_t bytes() const {
     return atab.bytes + hpix.bytes + hrange.bytes + dz.bytes + iz.bytes +
-           vz.bytes + dconv.bytes + offs.bytes + rays.bytes + ents.bytes +
+           vz.bytes + dconv.bytes + offs.bytes + rays.bytes + ents.bytes + alim.bytes +
            dpix.bytes + err.bytes;
   }
 };

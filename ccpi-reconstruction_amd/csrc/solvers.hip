@@ -95,7 +95,9 @@ __global__ __launch_bounds__(kRedThreads) void cgls_alpha_kernel(const Acc *part
       const float a = sc->gamma_f / (float)s;
       sc->alpha_f = a;
     } else {
-      sc->alpha_f = (float)(sc->gamma / (double)s);
+      // a zero direction (b in the null space of A^T) stops instead of
+      // producing 0/0 as the float reference does
+      sc->alpha_f = (double)s > 0.0 ? (float)(sc->gamma / (double)s) : 0.f;
     }
   }
 }
@@ -116,7 +118,7 @@ __global__ __launch_bounds__(kRedThreads) void cgls_beta_kernel(const Acc *part,
       norms[it] = (double)sqrtf(nn);
     } else {
       const double nn = (double)s;
-      sc->beta_f = (float)(nn / sc->gamma);
+      sc->beta_f = sc->gamma > 0.0 ? (float)(nn / sc->gamma) : 0.f;
       sc->gamma = nn;
       norms[it] = sqrt(nn);
     }

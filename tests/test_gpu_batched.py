@@ -35,11 +35,13 @@ def test_batched_cgls_matches_oracle_and_plain():
     bs = []
     for i, c in enumerate(cases):
         ph = ellipsoid_phantom(c.shape, c.vsize, c.origin, 3, seed=30 + i)
+        ph += 0.05 * np.random.default_rng(30 + i).random(c.shape).astype(np.float32)
         bs.append(oracle_geom(c).fp(ph))
     geoms = [_g(c) for c in cases]
     dev = [torch.from_numpy(b).cuda() for b in bs]
     xs, rs, norms = cgls_batched(geoms, dev, 5)
     for c, g, b, x, r, n in zip(cases, geoms, bs, xs, rs, norms):
+        assert np.abs(b).max() > 0
         xo, ro, no = oracle.cgls(oracle_geom(c), b, 5)
         scale = np.abs(xo).max()
         assert np.abs(x.cpu().numpy() - xo).max() < 1e-4 * scale
@@ -55,11 +57,27 @@ def test_batched_cgls_repeatable():
     from ccpi.reconstruction.projectors import cgls_batched
     cases = _cases()[:3]
     geoms = [_g(c) for c in cases]
-    bs = [torch.from_numpy(oracle_geom(c).fp(
-        ellipsoid_phantom(c.shape, c.vsize, c.origin, 3, seed=40 + i))).cuda()
-        for i, c in enumerate(cases)]
+    rng = np.random.default_rng(40)
+    bs = [torch.from_numpy(oracle_geom(c).fp(rng.random(c.shape).astype(np.float32))).cuda()
+          for c in cases]
     xa, _, na = cgls_batched(geoms, bs, 4)
     xb, _, nb = cgls_batched(geoms, bs, 4)
     for a, b in zip(xa, xb):
         assert torch.equal(a, b)
     assert np.array_equal(na, nb)
+
+
+def test_zero_sinogram_stops_cleanly():
+    """A problem whose data are all zero (A^T b = 0) keeps x = 0 with FP64
+    reductions, in the batched and in the one-problem form; the float
+    reference would divide 0 by 0 there."""
+    from ccpi.reconstruction.projectors import cgls, cgls_batched
+    cases = _cases()[:2]
+    geoms = [_g(c) for c in cases]
+    bs = [torch.zeros(c.sino_shape, dtype=torch.float32, device="cuda") for c in cases]
+    xs, _, norms = cgls_batched(geoms, bs, 3)
+    for x in xs:
+        assert torch.count_nonzero(x) == 0
+    assert np.all(norms == 0)
+    x, _, n = cgls(geoms[0], bs[0], 3)
+    assert torch.count_nonzero(x) == 0 and np.all(n == 0)
