@@ -56,6 +56,21 @@ struct ConeEnt {
   float alpha;
 };
 
+// Backprojection list of one voxel column: its (angle, h) rays in angle-
+// then-h order, and the column's row-set bounds (c2D.cpp:657-676).
+struct alignas(32) BpCol {
+  uint64_t off;   // first record in the record table
+  int n;          // number of records
+  int min_v;      // rows >= min_v stay above the bottom for every ray
+  int max_v;      // rows < max_v stay below the top for every ray
+  int pad[3];
+};
+
+// One record (16 bytes): x = ray (a * nh + h), y / z = alpha interval of
+// the ray inside the column (float bits), w = vlo | vhi << 16, the rows
+// whose ray is still inside the volume's z range at the interval start.
+using BpRec = uint4;
+
 // Kernel argument block of the cone projectors (by value).
 struct ConeArgs {
   int na, nh, nv, nx, ny, nz;
@@ -63,6 +78,8 @@ struct ConeArgs {
   int bp_stride;         // BP row interleave (>= rows per slice)
   int bp_unique;         // stride guarantees distinct slices per instruction
   int bp_main_safe;      // source inside the volume's z range
+  int bp_atomic;         // BP accumulates with LDS atomics (one array)
+  int nvp;               // row stride of the BP's scaled sinogram (padded)
   float zb, zt;          // volume bottom / top relative to source z
   const float *alim;     // [nv] walk parameter where row v leaves the volume's
                          // z range (>= 2 where it never does)

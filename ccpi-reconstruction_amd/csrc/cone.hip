@@ -230,16 +230,102 @@ __device__ inline float buf_load_f32(__amdgpu_buffer_rsrc_t r, int byte_off) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
 }
 
+// v_med3_i32: x clamped to [lo, hi] in one instruction (lo <= hi).
+__device__ inline int med3_i32(int x, int lo, int hi) {
+  int r;
+  asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
+  return r;
+}
+
+// min(a, b) for a uniform a as one v_min_f32 (the select form needs a
+// copy of a into a VGPR); operands are finite alphas, so this is the
+// reference's `a < b ? a : b`.
+__device__ inline float min_sv(float a, float b) {
+  float r;
+  asm("v_min_f32 %0, %1, %2" : "=v"(r) : "s"(a), "v"(b));
+  return r;
+}
+
+// Per-lane constants of one detector row in the FP segment loop.
+struct FpRow {
+  float dzv, izv;      // row's z run per unit alpha and its inverse
+  float kb;            // slice base: pzbz, or pzbz + 1 for a floor-form row
+  int ks;              // 1 for a floor-form row (c2D.cpp:153-167), else 0
+  int klo, khi;        // clamp of k for the boundary plane (shifted above)
+  const float *zb;     // s_vz, or s_vz + 1 above the source
+  int dir4;            // byte step to the second slice: -4 below, +4 above
+};
+
+// One xy segment [a0, a1] of column `col`, ai = a0 / voxel z size: the
+// row's two slices (c2D.cpp:104-143, 153-187) weighted by their alpha
+// lengths.  Slices outside [0, nz) read 0 through the buffer range check.
+template <bool EDGE>
+__device__ inline void fp_seg(const FpRow &R, float a0, float a1, float ai,
+                              __amdgpu_buffer_rsrc_t col, float &c0, float &c1,
+                              float &w0, float &w1) {
+  int k = (int)(R.kb + ai * R.dzv);
+  if (EDGE) k -= R.ks;
+  const float az = R.zb[med3_i32(k, R.klo, R.khi)] * R.izv;
+  const float mz = min_sv(a1, az);
+  w0 = mz - a0;
+  w1 = a1 - mz;
+  const int off = k * 4;
+  c0 = buf_load_f32(col, off);
+  c1 = buf_load_f32(col, off + R.dir4);
+}
+
+__device__ inline __amdgpu_buffer_rsrc_t column_rsrc(const float *vol, uint32_t col, int nz) {
+  const uint32_t c = __builtin_amdgcn_readfirstlane(col);
+  return __builtin_amdgcn_make_buffer_rsrc((void *)(vol + (size_t)c * nz), (short)0, nz * 4,
+                                           kRsrcWord3);
+}
+
+constexpr int kFpGroup = 4;
+
+// Segments [mb, me) of one ray for the wave's rows.  Groups of kFpGroup
+// segments issue all their gathers before the in-order accumulation (the
+// sum order is the reference's).  The wave stops once the segment start
+// passes every row's exit parameter (amax_bits: a non-negative float's
+// bits, so the integer compare is the float compare).
+template <bool EDGE>
+__device__ inline float fp_run(const FpRow &R, const ConeEnt *__restrict__ e,
+                               const float *__restrict__ ai, const float *vol, int nz,
+                               int mb, int me, int amax_bits, float acc) {
+  int m = mb;
+  for (; m + kFpGroup <= me; m += kFpGroup) {
+    if (__float_as_int(e[m - 1].alpha) >= amax_bits) return acc;
+    float c0[kFpGroup], c1[kFpGroup], w0[kFpGroup], w1[kFpGroup];
+#pragma unroll
+    for (int j = 0; j < kFpGroup; ++j) {
+      const ConeEnt en = e[m + j];
+      fp_seg<EDGE>(R, e[m + j - 1].alpha, en.alpha, ai[m + j], column_rsrc(vol, en.col, nz),
+                   c0[j], c1[j], w0[j], w1[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kFpGroup; ++j) acc = acc + (c0[j] * w0[j] + c1[j] * w1[j]);
+  }
+  for (; m < me; ++m) {
+    if (__float_as_int(e[m - 1].alpha) >= amax_bits) return acc;
+    const ConeEnt en = e[m];
+    float c0, c1, w0, w1;
+    fp_seg<EDGE>(R, e[m - 1].alpha, en.alpha, ai[m], column_rsrc(vol, en.col, nz), c0, c1,
+                 w0, w1);
+    acc = acc + (c0 * w0 + c1 * w1);
+  }
+  return acc;
+}
+
+// One workgroup per ray and 256 detector rows, one row per lane.
 __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
     ConeArgs g, const ConeRay *__restrict__ rays, const ConeEnt *__restrict__ ents,
-    const float *__restrict__ vol, float *__restrict__ sino) {
+    const float *__restrict__ eai, const float *__restrict__ vol, float *__restrict__ sino) {
   extern __shared__ float s_vz[];
   for (int q = threadIdx.x; q <= g.nz; q += kFpRows) s_vz[q] = g.vz[q];
   __syncthreads();
   const int r = blockIdx.x;
   const int v0 = blockIdx.y * kFpRows + threadIdx.x;
   // lanes past the last row mirror row nv - 1 and store nothing, so the
-  // wave-wide reductions below see every lane
+  // wave-wide reduction below sees every lane
   const bool live = v0 < g.nv;
   const int v = live ? v0 : g.nv - 1;
   const int h = r % g.nh;
@@ -252,39 +338,40 @@ __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
   float amax = g.alim[v];
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) amax = fmaxf(amax, __shfl_xor(amax, d, 64));
+  const int amax_bits = __builtin_amdgcn_readfirstlane(__float_as_int(amax));
   if (R.n > 1) {
-    const float dzv = g.dz[v], izv = g.iz[v];
-    const bool lower = v < g.mid;
-    const int dir = lower ? -1 : 1;
     const int nz = g.nz;
+    const bool lower = v < g.mid;
+    FpRow row;
+    row.dzv = g.dz[v];
+    row.izv = g.iz[v];
+    row.klo = lower ? 0 : -1;
+    row.khi = lower ? nz : nz - 1;
+    row.zb = s_vz + (lower ? 0 : 1);
+    row.dir4 = lower ? -4 : 4;
+    row.kb = g.pzbz;
+    row.ks = 0;
     const ConeEnt *__restrict__ e = ents + R.off;
-    float a0 = e[0].alpha;
-    for (int m = 1; m < R.n; ++m) {
-      if (a0 >= amax) break;
-      const ConeEnt en = e[m];
-      const float a1 = en.alpha;
-      const float ai = a0 * g.inv_dz;
-      // rows leaving through the bottom face use the floor-safe form
-      // (c2D.cpp:153-167); all other rows c2D.cpp:104-143, 175-188
-      const bool edge = (m >= R.ncom) && (v < R.min_v);
-      const int k = (int)((edge ? g.pzbz1 : g.pzbz) + ai * dzv) - (edge ? 1 : 0);
-      const int kz = min(max(lower ? k : k + 1, 0), nz);
-      const float az = s_vz[kz] * izv;
-      const float mz = a1 < az ? a1 : az;
-      const float w0 = mz - a0, w1 = a1 - mz;
-      const int k2 = k + dir;
-      // the column as a buffer resource (base in SGPRs): slices outside
-      // [0, nz) read 0 through the hardware range check
-      const uint32_t colid = __builtin_amdgcn_readfirstlane(en.col);
-      const __amdgpu_buffer_rsrc_t col = __builtin_amdgcn_make_buffer_rsrc(
-          (void *)(vol + (size_t)colid * nz), (short)0, nz * 4, kRsrcWord3);
-      const float c0 = buf_load_f32(col, k * 4);
-      const float c1 = buf_load_f32(col, k2 * 4);
-      acc = acc + (c0 * w0 + c1 * w1);
-      a0 = a1;
-    }
+    const float *__restrict__ ai = eai + R.off;
+    // segments before ncom: every row uses the truncating form
+    const int ncom = min(R.ncom, R.n);
+    acc = fp_run<false>(row, e, ai, vol, nz, 1, ncom, amax_bits, acc);
+    // from ncom on, rows below min_v use the floor form (c2D.cpp:153-167)
+    const bool edge = v < R.min_v;
+    row.kb = edge ? g.pzbz1 : g.pzbz;
+    row.ks = edge ? 1 : 0;
+    acc = fp_run<true>(row, e, ai, vol, nz, ncom, R.n, amax_bits, acc);
   }
   if (live) *out = acc * g.dconv[(size_t)h * g.nv + v];
+}
+
+// eai[q] = ents[q - 1].alpha / voxel z size: the slice-position factor of
+// the segment ending at entry q (the same float product the FP would form).
+__global__ void cone_ai_kernel(const ConeEnt *__restrict__ ents, uint64_t n, float inv_dz,
+                               float *__restrict__ eai) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+    eai[q] = q ? ents[q - 1].alpha * inv_dz : 0.0f;
 }
 
 // Counts the nonzero-weight voxel operands of cone_fp_kernel (same row
@@ -332,7 +419,6 @@ __global__ __launch_bounds__(kFpRows) void cone_nnz_kernel(
 // ---------------------------------------------------------------------------
 
 constexpr int kBpWaves = 4;
-constexpr int kBpCap = 256;  // LDS entries per wave per batch
 
 // Detector columns covered by the four corner lines of one voxel column
 // (c2D.cpp:837-901).
@@ -398,23 +484,9 @@ __device__ inline int wave_excl_scan(int x, int lane, int &total) {
   return incl - x;
 }
 
-// One LDS entry of a column's ray list.
-struct BpEnt {
-  uint32_t ray;   // a * nh + h
-  float a0, a1;   // alpha interval inside the column
-  float ai;       // a0 / voxel z size
-  int vlo, vhi;   // rows still inside the volume's z range at a0
-  int pad0, pad1;
-};
-
 // Rows whose ray is inside the volume's z range at walk parameter a0:
 // alim is increasing below the source row and decreasing above it.
 __device__ inline void live_rows(const ConeArgs &g, float a0, int &vlo, int &vhi) {
-  if (!g.bp_main_safe) {
-    vlo = 0;
-    vhi = g.nv;
-    return;
-  }
   int lo = 0, hi = g.mid;  // first lower row with alim > a0
   while (lo < hi) {
     const int m = (lo + hi) >> 1;
@@ -432,36 +504,36 @@ __device__ inline void live_rows(const ConeArgs &g, float a0, int &vlo, int &vhi
 
 __device__ inline int lds_pad4(int n) { return (n + 3) & ~3; }
 
-__global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
-    ConeArgs g, const float *__restrict__ dpix, float *__restrict__ vol,
-    int *__restrict__ err) {
-  extern __shared__ float smem[];
-  float *s_vz = smem;
-  const int nz = g.nz, nv = g.nv;
-  for (int q = threadIdx.x; q <= nz; q += 64 * kBpWaves) s_vz[q] = g.vz[q];
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int colid = blockIdx.x * kBpWaves + wave;
-  if (colid >= g.nx * g.ny) return;
-  float *acc = smem + lds_pad4(nz + 1) + wave * (lds_pad4(nz) + 8 * kBpCap);
-  BpEnt *ent = (BpEnt *)(acc + lds_pad4(nz));
+// Column extent in world units.
+struct ColBox {
+  double x0, xn, y0, yn;
+};
+__device__ inline ColBox col_box(const ConeArgs &g, int colid) {
   const int i = colid / g.ny, j = colid - (colid / g.ny) * g.ny;
-  const double x0 = g.vx + (double)i * g.dx, xn = g.vx + (double)(i + 1) * g.dx;
-  const double y0 = g.vy + (double)j * g.dy, yn = g.vy + (double)(j + 1) * g.dy;
-  float *__restrict__ vcol = vol + (size_t)colid * nz;
+  return ColBox{g.vx + (double)i * g.dx, g.vx + (double)(i + 1) * g.dx,
+                g.vy + (double)j * g.dy, g.vy + (double)(j + 1) * g.dy};
+}
 
-  // pass 1: rows that stay inside for every ray (c2D.cpp:657-676)
+// Plan pass 1, one wavefront per column, lanes over angles: the number of
+// (angle, h) rays crossing the column and the row-set bounds min_v / max_v
+// (c2D.cpp:657-676).
+__global__ __launch_bounds__(256) void cone_bpcount_kernel(ConeArgs g, BpCol *__restrict__ cols) {
+  const int lane = threadIdx.x & 63;
+  const int colid = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (colid >= g.nx * g.ny) return;
+  const ColBox B = col_box(g, colid);
   const float pzdeps = g.pzdv * (float)(1.0 + kEps);
-  int lo_max = 0, hi_min = nv;
+  int lo_max = 0, hi_min = g.nv, cnt = 0;
   for (int a = lane; a < g.na; a += 64) {
     const double *t = g.atab + 8 * a;
     int hmin, hmax;
-    cone_col_hrange(g, t, x0, xn, y0, yn, hmin, hmax);
+    cone_col_hrange(g, t, B.x0, B.xn, B.y0, B.yn, hmin, hmax);
     for (int h = hmin; h <= hmax; h++) {
       float a0, a1;
-      if (cone_col_alpha(g, t, h, x0, xn, y0, yn, a0, a1)) {
+      if (cone_col_alpha(g, t, h, B.x0, B.xn, B.y0, B.yn, a0, a1)) {
         lo_max = max(lo_max, (int)ceilf(pzdeps + g.z1 / a0));
         hi_min = min(hi_min, (int)floorf(g.pzdv + g.znm / a0));
+        ++cnt;
       }
     }
   }
@@ -469,160 +541,231 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
   for (int d = 32; d >= 1; d >>= 1) {
     lo_max = max(lo_max, __shfl_xor(lo_max, d, 64));
     hi_min = min(hi_min, __shfl_xor(hi_min, d, 64));
+    cnt += __shfl_xor(cnt, d, 64);
   }
-  const int min_v = lo_max, max_v = hi_min, mid = g.mid;
-  // every row in exactly one of the four row sets of c2D.cpp:687-749?
-  const bool one_set = (min_v <= mid) && (max_v >= mid);
+  if (lane == 0) {
+    BpCol c{};
+    c.n = cnt;
+    c.min_v = lo_max;
+    c.max_v = hi_min;
+    cols[colid] = c;
+  }
+}
 
-  for (int k = lane; k < nz; k += 64) acc[k] = vcol[k];
-
-  const int S = g.bp_stride;
-  // pass 2: batches of angles -> LDS entry list -> row sweep
+// Plan pass 2: the records of each column in angle-then-h order.
+__global__ __launch_bounds__(256) void cone_bpfill_kernel(ConeArgs g, const BpCol *__restrict__ cols,
+                                                          BpRec *__restrict__ recs) {
+  const int lane = threadIdx.x & 63;
+  const int colid = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (colid >= g.nx * g.ny) return;
+  const ColBox B = col_box(g, colid);
+  const BpCol C = cols[colid];
+  // Rows past their exit parameter add nothing when every row is in one
+  // set and the source is inside the z range; otherwise the window is all
+  // rows (with overlapping sets a row can take a lower-set form above the
+  // source, c2D.cpp:687-749).
+  const bool cut = g.bp_main_safe && C.min_v <= g.mid && C.max_v >= g.mid;
+  BpRec *out = recs + C.off;
+  int base_pos = 0;
   for (int base = 0; base < g.na; base += 64) {
     const int a = base + lane;
     int hmin = 0, hmax = -1;
     const double *t = g.atab + 8 * (a < g.na ? a : 0);
-    if (a < g.na) cone_col_hrange(g, t, x0, xn, y0, yn, hmin, hmax);
+    if (a < g.na) cone_col_hrange(g, t, B.x0, B.xn, B.y0, B.yn, hmin, hmax);
     int cnt = 0;
     for (int h = hmin; h <= hmax; h++) {
       float a0, a1;
-      cnt += cone_col_alpha(g, t, h, x0, xn, y0, yn, a0, a1) ? 1 : 0;
+      cnt += cone_col_alpha(g, t, h, B.x0, B.xn, B.y0, B.yn, a0, a1) ? 1 : 0;
     }
-    if (cnt > kBpCap) {  // one angle alone overflows a batch
-      atomicMax(err, 1);
-      cnt = 0;
-      hmax = hmin - 1;
+    int total;
+    int q = base_pos + wave_excl_scan(cnt, lane, total);
+    for (int h = hmin; h <= hmax; h++) {
+      float a0, a1;
+      if (cone_col_alpha(g, t, h, B.x0, B.xn, B.y0, B.yn, a0, a1)) {
+        int vlo = 0, vhi = g.nv;
+        if (cut) live_rows(g, a0, vlo, vhi);
+        out[q] = BpRec{(uint32_t)(a * g.nh + h), __builtin_bit_cast(uint32_t, a0),
+                       __builtin_bit_cast(uint32_t, a1),
+                       (uint32_t)vlo | ((uint32_t)vhi << 16)};
+        ++q;
+      }
     }
-    bool pending = true;
-    while (__any(pending)) {
-      int total;
-      const int my = pending ? cnt : 0;
-      const int off = wave_excl_scan(my, lane, total);
-      const bool take = pending && (off + my <= kBpCap);
-      int used;
-      const int off2 = wave_excl_scan(take ? my : 0, lane, used);
-      if (take) {
-        int q = off2;
-        for (int h = hmin; h <= hmax; h++) {
-          float a0, a1;
-          if (cone_col_alpha(g, t, h, x0, xn, y0, yn, a0, a1)) {
-            int vlo, vhi;
-            live_rows(g, a0, vlo, vhi);
-            ent[q] = BpEnt{(uint32_t)(a * g.nh + h), a0, a1, a0 * g.inv_dz, vlo, vhi, 0, 0};
-            ++q;
+    base_pos += total;
+  }
+}
+
+// Per-lane constants of one row slot of the sweep: lane L of slot (vb, t)
+// takes row vb + S*L + t.
+struct BpSlot {
+  float dzv, izv;  // row's z run per unit alpha and its inverse
+  float kb;        // slice base of the row's set; kBpOff for lanes not in it
+  int ks;          // 1 for the floor-form set (c2D.cpp:687-703), else 0
+  int dir;         // second slice: -1 for the lower sets, +1 for the upper
+};
+
+// A slice base that sends k (and k +- 1) far below 0: the contribution
+// lands in the trash cell.
+constexpr float kBpOff = -1.0e9f;
+
+// Row sets of c2D.cpp:687-749: 0 lower edge (v < min_v, floor form),
+// 1 lower main, 2 upper main, 3 upper edge.  With one_set every row is in
+// exactly one set; otherwise pass p sweeps set p over its rows.
+__device__ inline BpSlot bp_slot(const ConeArgs &g, int v, int pass, bool one_set,
+                                 int min_v, int max_v) {
+  BpSlot s;
+  const bool valid = v < g.nv;
+  const int vv = valid ? v : g.nv - 1;
+  const int mid = g.mid;
+  int set;
+  bool member;
+  if (one_set) {
+    set = v < min_v ? 0 : (v < mid ? 1 : (v < max_v ? 2 : 3));
+    member = valid;
+  } else {
+    set = pass;
+    member = valid && ((set == 0)   ? (v < min_v)
+                       : (set == 1) ? (v >= min_v && v < mid)
+                       : (set == 2) ? (v >= mid && v < max_v)
+                                    : (v >= max_v));
+  }
+  s.dzv = g.dz[vv];
+  s.izv = g.iz[vv];
+  s.kb = member ? (set == 0 ? g.pzbz1 : g.pzbz) : kBpOff;
+  s.ks = set == 0 ? 1 : 0;
+  s.dir = set <= 1 ? -1 : 1;
+  return s;
+}
+
+// Backprojection, one wavefront per voxel column: the column's list is
+// swept once per block of 64*S detector rows, the block's S row slots (lane
+// L of slot t takes row vb + S*L + t) holding their row constants in
+// registers (c2D.cpp:657-749 row sets, 1105-1114 weights).  A record skips
+// the blocks outside its live row window.
+//
+// Accumulation: S bounds the rows per slice, so the 64 rows of one slot
+// fall on distinct slices, and so do their second slices; slot t has its own
+// LDS array, so a slot's first-slice update is a plain read-add-write over
+// distinct cells, followed by its second-slice update.  Cells 0 and nz + 1
+// of an array are trash for slices outside the volume.  ATOMIC (rows per
+// slice beyond the stride bound, or arrays beyond the LDS budget): S = 1,
+// one array, ds_add_f32.
+template <int S, bool ATOMIC>
+__global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
+    ConeArgs g, const BpCol *__restrict__ cols, const BpRec *__restrict__ recs,
+    const float *__restrict__ dpix, float *__restrict__ vol) {
+  static_assert(!ATOMIC || S == 1, "atomic accumulation uses one array");
+  extern __shared__ float smem[];
+  const int nz = g.nz, nv = g.nv;
+  // slice planes with plane 0 repeated in front: s_vzx[q + 1] = plane max(q, 0)
+  float *s_vzx = smem;
+  for (int q = threadIdx.x; q <= nz + 1; q += 64 * kBpWaves) s_vzx[q] = g.vz[max(q - 1, 0)];
+  __syncthreads();
+  const float *s_vz1 = s_vzx + 1;
+  const int astride = lds_pad4(nz + 2);
+  // the wave index is uniform: keep the column's loads scalar
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int colid = blockIdx.x * kBpWaves + wave;
+  if (colid >= g.nx * g.ny) return;
+  const BpCol C = cols[colid];
+  if (C.n == 0) return;  // nothing crosses the column: it stays as it is
+  float *acc = smem + lds_pad4(nz + 2) + wave * S * astride;
+  float *__restrict__ vcol = vol + (size_t)colid * nz;
+  for (int q = lane; q < S * astride; q += 64) acc[q] = 0.0f;
+  __builtin_amdgcn_wave_barrier();
+  for (int k = lane; k < nz; k += 64) acc[k + 1] = vcol[k];
+  __builtin_amdgcn_wave_barrier();
+
+  const int min_v = C.min_v, max_v = C.max_v;
+  const bool one_set = (min_v <= g.mid) && (max_v >= g.mid);
+  const int npass = one_set ? 1 : 4;
+  constexpr int span = 64 * S;
+  const uint32_t lane_off = 4u * (uint32_t)(S * lane);
+  const float inv_dz = g.inv_dz;
+  const BpRec *__restrict__ R = recs + C.off;
+  const int nvp4 = 4 * g.nvp;
+  for (int vb = 0; vb < nv; vb += span) {
+    for (int pass = 0; pass < npass; ++pass) {
+      BpSlot sl[S];
+#pragma unroll
+      for (int t = 0; t < S; ++t) sl[t] = bp_slot(g, vb + S * lane + t, pass, one_set, min_v, max_v);
+      for (int e = 0; e < C.n; ++e) {
+        const BpRec rec = R[e];
+        const int vlo = (int)(rec.w & 0xffffu), vhi = (int)(rec.w >> 16);
+        if (vb >= vhi || vb + span <= vlo) continue;  // block outside the live rows
+        const float a0 = __builtin_bit_cast(float, rec.y);
+        const float a1 = __builtin_bit_cast(float, rec.z);
+        const float ai = a0 * inv_dz;
+        const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(dpix + (size_t)rec.x * g.nvp), (short)0, nvp4, kRsrcWord3);
+        float p[S];
+#pragma unroll
+        for (int t = 0; t < S; ++t)
+          p[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                              prow, (int)lane_off, 4 * (vb + t), 0));
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+          const int k = (int)(sl[t].kb + ai * sl[t].dzv) - sl[t].ks;
+          const int i0 = med3_i32(k, -1, nz), i1 = med3_i32(k + sl[t].dir, -1, nz);
+          // boundary plane between the two slices (k below the source,
+          // k + 1 above it); clamping the larger index equals clamping the
+          // plane index to [0, nz]
+          const float az = s_vz1[max(i0, i1)] * sl[t].izv;
+          const float mz = min_sv(a1, az);
+          const float c0 = p[t] * (mz - a0), c1 = p[t] * (a1 - mz);
+          float *ac = acc + (ATOMIC ? 0 : t * astride) + 1;
+          if (ATOMIC) {
+            atomicAdd(&ac[i0], c0);
+            atomicAdd(&ac[i1], c1);
+          } else {
+            ac[i0] = ac[i0] + c0;
+            __builtin_amdgcn_wave_barrier();
+            ac[i1] = ac[i1] + c1;
+            __builtin_amdgcn_wave_barrier();
           }
         }
-        pending = false;
       }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      // Sweep the batch: row block outer, entries inner.  Lane L takes rows
-      // vb + S*L + t (t < S).  S bounds the rows per slice, so the lanes of
-      // one instruction hit distinct slices and the column update is a
-      // plain LDS read-add-write (bp_unique); otherwise ds_add.
-      for (int vb = 0; vb < nv; vb += 64 * S) {
-        for (int tt = 0; tt < S; ++tt) {
-          const int v = vb + S * lane + tt;
-          const int vfirst = vb + tt, vlast = vb + tt + 63 * S;
-          const bool live = v < nv;
-          const int vv = live ? v : nv - 1;
-          const float dzv = g.dz[vv], izv = g.iz[vv];
-          const bool lower = v < mid;
-          const int kzoff = lower ? 0 : 1, dir = lower ? -1 : 1;
-          if (g.bp_main_safe && g.bp_unique && one_set && vfirst >= min_v && vlast < max_v) {
-            // every lane a main row inside the volume (min_v / max_v bound
-            // the slices, c2D.cpp:657-676): no range checks
-            for (int e = 0; e < used; ++e) {
-              const BpEnt E = ent[e];
-              if (vfirst >= E.vhi || vlast < E.vlo) continue;  // rows all outside
-              const uint32_t ray = __builtin_amdgcn_readfirstlane(E.ray);
-              const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
-                  (void *)(dpix + (size_t)ray * nv), (short)0, nv * 4, kRsrcWord3);
-              const float p = buf_load_f32(prow, v * 4);
-              // clamps are insurance only: main rows stay in [0, nz)
-              const int k = min(max((int)(g.pzbz + E.ai * dzv), 0), nz - 1);
-              const int k2 = min(max(k + dir, 0), nz - 1);
-              const float az = s_vz[k + kzoff] * izv;
-              const float mz = E.a1 < az ? E.a1 : az;
-              acc[k] = acc[k] + p * (mz - E.a0);
-              __builtin_amdgcn_wave_barrier();
-              acc[k2] = acc[k2] + p * (E.a1 - mz);
-              __builtin_amdgcn_wave_barrier();
-            }
-            continue;
-          }
-          const int npass = one_set ? 1 : 4;
-          for (int pass = 0; pass < npass; ++pass) {
-            // row set: 0 lower edge, 1 lower main, 2 upper main, 3 upper edge
-            int set;
-            bool member;
-            if (one_set) {
-              set = v < min_v ? 0 : (v < mid ? 1 : (v < max_v ? 2 : 3));
-              member = live;
-            } else {
-              set = pass;
-              member = live && ((set == 0) ? (v < min_v)
-                              : (set == 1) ? (v >= min_v && v < mid)
-                              : (set == 2) ? (v >= mid && v < max_v)
-                                           : (v >= max_v));
-            }
-            if (!__any(member)) continue;
-            const bool low = set <= 1;
-            const float kbase = set == 0 ? g.pzbz1 : g.pzbz;
-            const int ksub = set == 0 ? 1 : 0;
-            const int zo = low ? 0 : 1, dr = low ? -1 : 1;
-            for (int e = 0; e < used; ++e) {
-              const BpEnt E = ent[e];
-              if (vfirst >= E.vhi || vlast < E.vlo) continue;  // rows all outside
-              const uint32_t ray = __builtin_amdgcn_readfirstlane(E.ray);
-              const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
-                  (void *)(dpix + (size_t)ray * nv), (short)0, nv * 4, kRsrcWord3);
-              const float p = buf_load_f32(prow, vv * 4);
-              const int k = (int)(kbase + E.ai * dzv) - ksub;
-              const int k2 = k + dr;
-              const bool ok0 = member && k >= 0 && k < nz;
-              const bool ok1 = member && k2 >= 0 && k2 < nz;
-              const int kz = min(max(k + zo, 0), nz);
-              const float az = s_vz[kz] * izv;
-              const float mz = E.a1 < az ? E.a1 : az;
-              const float c0 = p * (mz - E.a0), c1 = p * (E.a1 - mz);
-              if (g.bp_unique) {
-                if (ok0) acc[k] = acc[k] + c0;
-                __builtin_amdgcn_wave_barrier();
-                if (ok1) acc[k2] = acc[k2] + c1;
-                __builtin_amdgcn_wave_barrier();
-              } else {
-                if (ok0) atomicAdd(&acc[k], c0);
-                if (ok1) atomicAdd(&acc[k2], c1);
-              }
-            }
-          }
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     }
   }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  for (int k = lane; k < nz; k += 64) vcol[k] = acc[k];
+  for (int k = lane; k < nz; k += 64) {
+    float s = acc[k + 1];
+#pragma unroll
+    for (int t = 1; t < S; ++t) s += acc[t * astride + k + 1];
+    vcol[k] = s;
+  }
 }
 
-// dpix = pix * d_conv (c2D.cpp:1105-1114); also clears the BP overflow flag
-// (a kernel store, so the sequence stays graph-capturable).
+// dpix[r][v] = pix[r][v] * d_conv[h][v] for v < nv, 0 up to the padded row
+// stride nvp (c2D.cpp:1105-1114).
 __global__ void cone_scale_kernel(const float *__restrict__ pix,
                                   const float *__restrict__ dconv,
-                                  float *__restrict__ dpix, size_t total,
-                                  int nhv, int *__restrict__ err) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *err = 0;
+                                  float *__restrict__ dpix, size_t nrays, int nh, int nv,
+                                  int nvp) {
+  const size_t total = nrays * (size_t)nvp;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride)
-    dpix[q] = pix[q] * dconv[q % (size_t)nhv];
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+    const size_t r = q / (size_t)nvp;
+    const int v = (int)(q - r * (size_t)nvp);
+    const int h = (int)(r % (size_t)nh);
+    dpix[q] = v < nv ? pix[r * (size_t)nv + v] * dconv[(size_t)h * nv + v] : 0.0f;
+  }
 }
 
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+
+// largest row stride with read-add-write accumulation (one kernel each)
+constexpr int kBpMaxStride = 4;
+
+// LDS of one BP workgroup: the slice planes and `narr` column arrays per
+// wavefront.
+static size_t bp_lds_bytes(int nz, int narr) {
+  const size_t pad = (size_t)((nz + 2 + 3) & ~3);
+  return sizeof(float) * (pad + (size_t)kBpWaves * (size_t)narr * pad);
+}
 
 void cone_build_plan(ConePlan &P, hipStream_t st) {
   const Geometry &G = P.g;
@@ -655,6 +798,14 @@ void cone_build_plan(ConePlan &P, hipStream_t st) {
     iz[v] = (float)(1.0 / (double)dz[v]);
   }
   for (int k = 0; k <= nz; k++) vz[k] = (float)(G.origin[2] + (double)k * G.vsize[2] - sz);
+  {
+    // the FP forms byte offsets 4 * (slice +- 1) in 32-bit integers
+    double kmax = std::fabs((double)A.pzbz) + 2.0;
+    for (int v = 0; v < nv; v++) kmax = std::max(kmax, std::fabs((double)A.pzbz) + 2.0 +
+                                                           std::fabs((double)dz[v]) * A.inv_dz);
+    CCPI_REQUIRE(kmax < (double)(1 << 28), CCPI_E_LIMIT,
+                 "cone geometry: detector rows span too many voxel slices");
+  }
   const double x2 = (G.det_x - sx) * (G.det_x - sx);
   for (int h = 0; h < nh; h++) {
     const double xy2 = x2 + (G.h_pix[h] - sy) * (G.h_pix[h] - sy);
@@ -731,6 +882,17 @@ void cone_build_plan(ConePlan &P, hipStream_t st) {
       uniform = uniform && std::fabs((G.v_pix[v] - G.v_pix[v - 1]) - st0) <= 1e-9 * std::fabs(st0);
     A.bp_main_safe =
         (uniform && sz >= G.origin[2] && sz < G.origin[2] + (double)nz * G.vsize[2]) ? 1 : 0;
+    // one LDS array per row phase while they fit two workgroups per CU
+    // beside each other; otherwise LDS atomics on one array
+    A.bp_atomic =
+        (A.bp_unique && A.bp_stride <= kBpMaxStride && bp_lds_bytes(nz, A.bp_stride) <= 64 * 1024)
+            ? 0
+            : 1;
+    CCPI_REQUIRE(bp_lds_bytes(nz, A.bp_atomic ? 1 : A.bp_stride) <= 160 * 1024, CCPI_E_LIMIT,
+                 "cone BP: nz too large for LDS");
+    const int span = 64 * (A.bp_atomic ? 1 : A.bp_stride);
+    A.nvp = (nv + span - 1) / span * span;
+    CCPI_REQUIRE(nv < 65536, CCPI_E_LIMIT, "cone beam: at most 65535 detector rows");
   }
   // Walk parameter past which a row's ray is outside the volume's z range
   // for good: a row below the source is kLimSlices slices under the bottom
@@ -794,6 +956,35 @@ void cone_build_plan(ConePlan &P, hipStream_t st) {
                      P.offs.as<uint64_t>(), nullptr, P.rays.as<ConeRay>(),
                      P.ents.as<ConeEnt>());
   CCPI_HIP(hipGetLastError());
+  P.eai.alloc(std::max<uint64_t>(tot, 1) * sizeof(float));
+  hipLaunchKernelGGL(cone_ai_kernel, dim3(2048), dim3(256), 0, st, P.ents.as<ConeEnt>(),
+                     (uint64_t)tot, A.inv_dz, P.eai.as<float>());
+  CCPI_HIP(hipGetLastError());
+
+  // backprojection lists: per-column count and row-set bounds, scan, fill
+  {
+    const size_t ncols = (size_t)nx * ny;
+    P.bpcol.alloc(ncols * sizeof(BpCol));
+    const int cblocks = (int)((ncols * 64 + 255) / 256);
+    hipLaunchKernelGGL(cone_bpcount_kernel, dim3(cblocks), dim3(256), 0, st, A,
+                       P.bpcol.as<BpCol>());
+    CCPI_HIP(hipGetLastError());
+    std::vector<BpCol> hcol(ncols);
+    CCPI_HIP(hipMemcpyAsync(hcol.data(), P.bpcol.p, ncols * sizeof(BpCol),
+                            hipMemcpyDeviceToHost, st));
+    CCPI_HIP(hipStreamSynchronize(st));
+    uint64_t nrec = 0;
+    for (auto &c : hcol) {
+      c.off = nrec;
+      nrec += (uint64_t)c.n;
+    }
+    P.bpcol.upload(hcol, st);
+    P.bprec.alloc(std::max<uint64_t>(nrec, 1) * sizeof(BpRec));
+    hipLaunchKernelGGL(cone_bpfill_kernel, dim3(cblocks), dim3(256), 0, st, A,
+                       P.bpcol.as<BpCol>(), P.bprec.as<BpRec>());
+    CCPI_HIP(hipGetLastError());
+    CCPI_HIP(hipStreamSynchronize(st));  // hcol is the upload's source
+  }
   CCPI_HIP(hipStreamSynchronize(st));
   P.args = A;
 }
@@ -803,7 +994,8 @@ void cone_fp(const ConePlan &P, const float *vol, float *sino, hipStream_t st) {
   dim3 grid((unsigned)(A.na * A.nh), (unsigned)((A.nv + kFpRows - 1) / kFpRows));
   const size_t lds = sizeof(float) * (size_t)(A.nz + 1);
   hipLaunchKernelGGL(cone_fp_kernel, grid, dim3(kFpRows), lds, st, A,
-                     P.rays.as<ConeRay>(), P.ents.as<ConeEnt>(), vol, sino);
+                     P.rays.as<ConeRay>(), P.ents.as<ConeEnt>(), P.eai.as<float>(), vol,
+                     sino);
   CCPI_HIP(hipGetLastError());
 }
 
@@ -816,30 +1008,39 @@ void cone_nnz(const ConePlan &P, unsigned long long *count, hipStream_t st) {
   CCPI_HIP(hipGetLastError());
 }
 
-size_t cone_bp_lds_bytes(int nz) {
-  const size_t pad1 = (size_t)((nz + 1 + 3) & ~3), pad0 = (size_t)((nz + 3) & ~3);
-  return sizeof(float) * (pad1 + (size_t)kBpWaves * (pad0 + 8 * kBpCap));
+size_t cone_dpix_floats(const ConePlan &P) {
+  return (size_t)P.args.na * P.args.nh * (size_t)P.args.nvp;
 }
 
-void cone_scale(const ConePlan &P, const float *pix, float *dpix, int *err,
-                hipStream_t st) {
+void cone_scale(const ConePlan &P, const float *pix, float *dpix, hipStream_t st) {
   const ConeArgs &A = P.args;
-  const size_t total = (size_t)A.na * A.nh * A.nv;
+  const size_t nrays = (size_t)A.na * A.nh;
+  const size_t total = nrays * (size_t)A.nvp;
   const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
-  hipLaunchKernelGGL(cone_scale_kernel, dim3(blocks), dim3(256), 0, st, pix,
-                     A.dconv, dpix, total, A.nh * A.nv, err);
+  hipLaunchKernelGGL(cone_scale_kernel, dim3(blocks), dim3(256), 0, st, pix, A.dconv, dpix,
+                     nrays, A.nh, A.nv, A.nvp);
   CCPI_HIP(hipGetLastError());
 }
 
-void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, int *err,
-                    hipStream_t st) {
+void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, hipStream_t st) {
   const ConeArgs &A = P.args;
-  const size_t lds = cone_bp_lds_bytes(A.nz);
-  CCPI_REQUIRE(lds <= 160 * 1024, CCPI_E_LIMIT, "cone BP: nz too large for LDS");
   const int cols = A.nx * A.ny;
   const int blocks = (cols + kBpWaves - 1) / kBpWaves;
-  hipLaunchKernelGGL(cone_bp_kernel, dim3(blocks), dim3(64 * kBpWaves), lds, st,
-                     A, dpix, vol, err);
+  const int S = A.bp_atomic ? 1 : A.bp_stride;
+  const size_t lds = bp_lds_bytes(A.nz, S);
+  const BpCol *bc = P.bpcol.as<BpCol>();
+  const BpRec *br = P.bprec.as<BpRec>();
+  const dim3 grid(blocks), block(64 * kBpWaves);
+  if (A.bp_atomic)
+    hipLaunchKernelGGL((cone_bp_kernel<1, true>), grid, block, lds, st, A, bc, br, dpix, vol);
+  else if (S == 1)
+    hipLaunchKernelGGL((cone_bp_kernel<1, false>), grid, block, lds, st, A, bc, br, dpix, vol);
+  else if (S == 2)
+    hipLaunchKernelGGL((cone_bp_kernel<2, false>), grid, block, lds, st, A, bc, br, dpix, vol);
+  else if (S == 3)
+    hipLaunchKernelGGL((cone_bp_kernel<3, false>), grid, block, lds, st, A, bc, br, dpix, vol);
+  else
+    hipLaunchKernelGGL((cone_bp_kernel<4, false>), grid, block, lds, st, A, bc, br, dpix, vol);
   CCPI_HIP(hipGetLastError());
 }
 

@@ -135,6 +135,9 @@ Cache &cache() {
   return c;
 }
 constexpr size_t kMaxPlans = 8;
+// plans beyond this total (the least recently used first) are dropped; the
+// newest plan always stays
+constexpr size_t kMaxPlanBytes = size_t(64) << 30;
 
 std::string dev_key(const Geometry &g) {
   int dev = 0;
@@ -154,6 +157,12 @@ Entry *lookup(Cache &c, const std::string &k) {
 void insert(Cache &c, const std::string &k, Entry e) {
   c.lru.emplace_front(k, std::move(e));
   while (c.lru.size() > kMaxPlans) c.lru.pop_back();
+  auto total = [&c] {
+    size_t b = 0;
+    for (auto &q : c.lru) b += q.second.bytes();
+    return b;
+  };
+  while (c.lru.size() > 1 && total() > kMaxPlanBytes) c.lru.pop_back();
 }
 }  // namespace
 
@@ -214,10 +223,9 @@ void Projector::fp(const float *vol, float *sino, hipStream_t st) const {
 void Projector::bp(const float *sino, float *vol, hipStream_t st) const {
   if (cone) {
     ConePlan &P = *cone;
-    P.dpix.alloc(P.g.n_pix() * sizeof(float));
-    P.err.alloc(sizeof(int));
-    cone_scale(P, sino, P.dpix.as<float>(), P.err.as<int>(), st);
-    cone_bp_scaled(P, P.dpix.as<float>(), vol, P.err.as<int>(), st);
+    P.dpix.alloc(cone_dpix_floats(P) * sizeof(float));
+    cone_scale(P, sino, P.dpix.as<float>(), st);
+    cone_bp_scaled(P, P.dpix.as<float>(), vol, st);
   } else {
     par_bp(*par, sino, vol, st);
   }

@@ -77,14 +77,15 @@ struct DeviceBuffer {
 struct ConePlan {
   Geometry g;
   ConeArgs args{};
-  DeviceBuffer atab, hpix, hrange, dz, iz, vz, dconv, offs, rays, ents, alim;
+  DeviceBuffer atab, hpix, hrange, dz, iz, vz, dconv, offs, rays, ents, eai, alim;
+  DeviceBuffer bpcol, bprec;  // backprojection lists
   DeviceBuffer dpix;    // scaled sinogram scratch of the BP
-  DeviceBuffer err;     // BP overflow flag
   uint64_t n_entries = 0;
   size_t bytes() const {
     return atab.bytes + hpix.bytes + hrange.bytes + dz.bytes + iz.bytes +
-           vz.bytes + dconv.bytes + offs.bytes + rays.bytes + ents.bytes + alim.bytes +
-           dpix.bytes + err.bytes;
+           vz.bytes + dconv.bytes + offs.bytes + rays.bytes + ents.bytes + eai.bytes + alim.bytes +
+           bpcol.bytes + bprec.bytes +
+           dpix.bytes;
   }
 };
 
@@ -105,11 +106,10 @@ long long plan_cache_bytes();
 // launchers (cone.hip, parallel.hip)
 void cone_build_plan(ConePlan &P, hipStream_t st);
 void cone_fp(const ConePlan &P, const float *vol, float *sino, hipStream_t st);
-void cone_scale(const ConePlan &P, const float *pix, float *dpix, int *err,
-                hipStream_t st);
-void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, int *err,
-                    hipStream_t st);
-size_t cone_bp_lds_bytes(int nz);
+// dpix holds na * nh rows of args.nvp floats (cone_dpix_floats)
+size_t cone_dpix_floats(const ConePlan &P);
+void cone_scale(const ConePlan &P, const float *pix, float *dpix, hipStream_t st);
+void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, hipStream_t st);
 
 void par_build_plan(ParPlan &P, hipStream_t st);
 void par_fp(const ParPlan &P, const float *vol, float *sino, hipStream_t st);
