@@ -221,6 +221,17 @@ __global__ __launch_bounds__(256) void cone_walk_kernel(
 // ---------------------------------------------------------------------------
 
 constexpr int kFpRows = 256;
+
+// Workgroups go to the 8 XCDs round-robin (workgroup id mod 8), each XCD
+// with its own L2.  xcd_block maps the hardware id to a logical id so that
+// every XCD runs one contiguous range of logical ids: neighbouring work
+// (adjacent voxel columns) then shares an L2.
+constexpr int kXcds = 8;
+__device__ inline int xcd_block(int bid, int n) {
+  const int xcd = bid % kXcds, local = bid / kXcds;
+  const int per = n / kXcds, rem = n % kXcds;
+  return xcd * per + min(xcd, rem) + local;
+}
 // dword 3 of a raw (untyped, stride 0) buffer resource on gfx9 CDNA
 constexpr int kRsrcWord3 = 0x00020000;
 
@@ -322,6 +333,8 @@ __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
   extern __shared__ float s_vz[];
   for (int q = threadIdx.x; q <= g.nz; q += kFpRows) s_vz[q] = g.vz[q];
   __syncthreads();
+  // (ray, row block); grouping rays per XCD measured slower here than the
+  // hardware's round-robin spread
   const int r = blockIdx.x;
   const int v0 = blockIdx.y * kFpRows + threadIdx.x;
   // lanes past the last row mirror row nv - 1 and store nothing, so the
@@ -665,8 +678,16 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
   // the wave index is uniform: keep the column's loads scalar
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
-  const int colid = blockIdx.x * kBpWaves + wave;
-  if (colid >= g.nx * g.ny) return;
+  // logical block -> 32 x 32 column tile (one XCD's resident set), then
+  // kBpWaves adjacent columns of one tile row
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  constexpr int kTile = 32, kTileBlocks = kTile * kTile / kBpWaves;
+  const int tile = lb / kTileBlocks, within = lb - tile * kTileBlocks;
+  const int ntj = (g.ny + kTile - 1) / kTile;
+  const int ci = (tile / ntj) * kTile + within / (kTile / kBpWaves);
+  const int cj = (tile % ntj) * kTile + (within % (kTile / kBpWaves)) * kBpWaves + wave;
+  if (ci >= g.nx || cj >= g.ny) return;
+  const int colid = ci * g.ny + cj;
   const BpCol C = cols[colid];
   if (C.n == 0) return;  // nothing crosses the column: it stays as it is
   float *acc = smem + lds_pad4(nz + 2) + wave * S * astride;
@@ -1024,8 +1045,8 @@ void cone_scale(const ConePlan &P, const float *pix, float *dpix, hipStream_t st
 
 void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, hipStream_t st) {
   const ConeArgs &A = P.args;
-  const int cols = A.nx * A.ny;
-  const int blocks = (cols + kBpWaves - 1) / kBpWaves;
+  // 32 x 32 column tiles of 32 * 32 / kBpWaves workgroups (kernel mapping)
+  const int blocks = ((A.nx + 31) / 32) * ((A.ny + 31) / 32) * (32 * 32 / kBpWaves);
   const int S = A.bp_atomic ? 1 : A.bp_stride;
   const size_t lds = bp_lds_bytes(A.nz, S);
   const BpCol *bc = P.bpcol.as<BpCol>();

@@ -103,7 +103,11 @@ def test_cgls_parity(name, dev):
     x, r, n = cgls(_geom(g), torch.from_numpy(b).to(dev), 5)
     assert _rel(x.cpu().numpy(), xo) < 1e-4
     assert _rel(r.cpu().numpy(), ro) < 1e-4
-    assert np.allclose(n, no, rtol=1e-4)
+    # residual norms against the FP64 oracle: the float oracle's partial
+    # sums are combined with OpenMP atomics (blas.hpp), so their low bits
+    # depend on the thread count
+    _, _, no64 = oracle.cgls(og, b.astype(np.float64), 5, dtype=np.float64)
+    assert np.allclose(n, no64, rtol=2e-4)
     # reference-precision reductions give the same iterates
     x32, _, n32 = cgls(_geom(g), torch.from_numpy(b).to(dev), 5, reduce_fp32=True)
     assert _rel(x32.cpu().numpy(), xo) < 1e-4
