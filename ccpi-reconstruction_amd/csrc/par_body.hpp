@@ -16,6 +16,7 @@ template <int RC>
 __device__ inline double par_fp_thread(const ParArgs &g, int bx, int by, int tid,
                                        const float *__restrict__ vol,
                                        float *__restrict__ sino, bool acc_in) {
+  static_assert(RC % 4 == 0, "rows are read in groups of four");
   const int r = bx * 256 + tid;
   if (r >= g.na * g.nh) return 0.0;
   const int a = r / g.nh, h = r - a * g.nh;
@@ -35,8 +36,30 @@ __device__ inline double par_fp_thread(const ParArgs &g, int bx, int by, int tid
   const double p2x = c * g.D - s * g.h_pix[h];
   const double p2y = s * g.D + c * g.h_pix[h];
 
+  // one row per slice (map_type 1): the lane's rows are the contiguous
+  // slices v0 .. v0 + nval - 1 of each column, read 16 bytes at a time
+  const int nval = g.map_type == 1 ? max(0, min(nrow, nz - v0)) : 0;
   auto add = [&](int x, int y, float l) {
     const float *__restrict__ col = vol + ((size_t)x * ny + y) * nz;
+    if (g.map_type == 1) {
+      const float *__restrict__ c = col + v0;
+#pragma unroll
+      for (int q4 = 0; q4 < RC / 4; q4++) {
+        if (4 * q4 + 3 < nval) {
+          float4 f;
+          __builtin_memcpy(&f, c + 4 * q4, sizeof(f));
+          acc[4 * q4] = acc[4 * q4] + f.x * l;
+          acc[4 * q4 + 1] = acc[4 * q4 + 1] + f.y * l;
+          acc[4 * q4 + 2] = acc[4 * q4 + 2] + f.z * l;
+          acc[4 * q4 + 3] = acc[4 * q4 + 3] + f.w * l;
+        } else {
+#pragma unroll
+          for (int q = 4 * q4; q < 4 * q4 + 4; q++)
+            if (q < nval) acc[q] = acc[q] + c[q] * l;
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < RC; q++)
       if (kmap[q] >= 0) acc[q] = acc[q] + col[kmap[q]] * l;
@@ -134,6 +157,7 @@ template <int RC>
 __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid,
                                        const float *__restrict__ sino,
                                        float *__restrict__ vol, bool acc_in) {
+  static_assert(RC % 4 == 0, "rows are read in groups of four");
   const int colid = bx * 256 + tid;
   if (colid >= g.nx * g.ny) return 0.0;
   const int i = colid / g.ny, j = colid - (colid / g.ny) * g.ny;
@@ -147,6 +171,7 @@ __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid
   const double x0 = g.ox + (double)i * g.dx, xn = g.ox + (double)(i + 1) * g.dx;
   const double y0 = g.oy + (double)j * g.dy, yn = g.oy + (double)(j + 1) * g.dy;
   const int T = g.map_type;
+  const int nvalid = max(0, min(nk, nv - k0));  // rows of the slices (T == 1)
 
   for (int a = 0; a < g.na; a++) {
     const double *t = g.atab + 5 * a;
@@ -214,9 +239,23 @@ __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid
       const float wf = (float)w;
       const float *__restrict__ p = sino + ((size_t)a * nh + h) * nv;
       if (T == 1) {
+        // rows k0 .. k0 + nvalid - 1 of the ray, read 16 bytes at a time
+        const float *__restrict__ pr = p + k0;
 #pragma unroll
-        for (int q = 0; q < RC; q++)
-          if (q < nk && k0 + q < nv) acc[q] = acc[q] + p[k0 + q] * wf;
+        for (int q4 = 0; q4 < RC / 4; q4++) {
+          if (4 * q4 + 3 < nvalid) {
+            float4 f;
+            __builtin_memcpy(&f, pr + 4 * q4, sizeof(f));
+            acc[4 * q4] = acc[4 * q4] + f.x * wf;
+            acc[4 * q4 + 1] = acc[4 * q4 + 1] + f.y * wf;
+            acc[4 * q4 + 2] = acc[4 * q4 + 2] + f.z * wf;
+            acc[4 * q4 + 3] = acc[4 * q4 + 3] + f.w * wf;
+          } else {
+#pragma unroll
+            for (int q = 4 * q4; q < 4 * q4 + 4; q++)
+              if (q < nvalid) acc[q] = acc[q] + pr[q] * wf;
+          }
+        }
       } else if (T == 2 || T == 4) {  // rows summed before scaling (p2D.cpp:694-761)
 #pragma unroll
         for (int q = 0; q < RC; q++) {
