@@ -41,6 +41,22 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel: str):
+    """HBM-side bytes per launch of `kernel` from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy); the
+    summary holds one `TRAFFIC <kernel> <bytes>` line per kernel."""
+    path = os.path.join(ROOT, "profiles", "cone512_projectors_counters.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "TRAFFIC" and parts[1] == kernel:
+                    return int(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(geom, phantom_np, n_sample: int):
     """Time the oracle's cone f2D + b2D on `n_sample` evenly spaced angles
     of the workload, scaled to all angles = one CGLS iteration."""
@@ -147,6 +163,9 @@ def main() -> int:
                     help="angles in the CPU-oracle sample (0 disables it)")
     ap.add_argument("--batched-iters", type=int, default=20,
                     help="CGLS iterations of the batched-vs-plain comparison (0 skips it)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device to device (the known-size kernel that "
+                         "scales FETCH_SIZE / WRITE_SIZE counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path, "
@@ -161,6 +180,12 @@ def main() -> int:
                                                 projector_nnz)
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     cfg = W.CONE_512
     geom = W.cone_geometry(**cfg)
     log("bench: phantom + sinogram")
@@ -231,7 +256,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom_name),
             "bytes_alg": bytes_alg,
             "nnz": nnz,
         },
