@@ -702,10 +702,60 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
   const int npass = one_set ? 1 : 4;
   constexpr int span = 64 * S;
   const uint32_t lane_off = 4u * (uint32_t)(S * lane);
-  const float inv_dz = g.inv_dz;
+  // 1 / voxel z size in a VGPR: a0 * inv_dz is then one v_mul with the
+  // record's a0 in an SGPR
+  float inv_dz;
+  asm("v_mov_b32 %0, %1" : "=v"(inv_dz) : "s"(g.inv_dz));
   const BpRec *__restrict__ R = recs + C.off;
   const int nvp4 = 4 * g.nvp;
   for (int vb = 0; vb < nv; vb += span) {
+    if (!ATOMIC && one_set && g.bp_main_safe && vb >= min_v && vb + span <= max_v) {
+      // Every row of the block is a main row: its ray stays inside the
+      // volume's z range for all of this column's rays (c2D.cpp:657-676), so
+      // k is in [0, nz) (the clamp only guards float ties) and k +- 1 in
+      // [-1, nz], where cells -1 and nz are trash.  No row-set tests.
+      float dzv[S], izv[S];
+      int d4[S];
+      const float *zb[S];
+#pragma unroll
+      for (int t = 0; t < S; ++t) {
+        const int v = vb + S * lane + t;
+        const bool lower = v < g.mid;
+        dzv[t] = g.dz[v];
+        izv[t] = g.iz[v];
+        d4[t] = lower ? -4 : 4;
+        zb[t] = s_vz1 + (lower ? 0 : 1);
+      }
+      for (int e = 0; e < C.n; ++e) {
+        const BpRec rec = R[e];
+        const int vlo = (int)(rec.w & 0xffffu), vhi = (int)(rec.w >> 16);
+        if (vb >= vhi || vb + span <= vlo) continue;  // block outside the live rows
+        const float a0 = __builtin_bit_cast(float, rec.y);
+        const float a1 = __builtin_bit_cast(float, rec.z);
+        const float ai = a0 * inv_dz;
+        const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(dpix + (size_t)rec.x * g.nvp), (short)0, nvp4, kRsrcWord3);
+        float p[S];
+#pragma unroll
+        for (int t = 0; t < S; ++t)
+          p[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                              prow, (int)lane_off, 4 * (vb + t), 0));
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+          const int k = med3_i32((int)(g.pzbz + ai * dzv[t]), 0, nz - 1);
+          float *c0p = acc + t * astride + 1 + k;
+          float *c1p = (float *)((char *)c0p + d4[t]);
+          const float az = zb[t][k] * izv[t];
+          const float mz = min_sv(a1, az);
+          const float c0 = p[t] * (mz - a0), c1 = p[t] * (a1 - mz);
+          *c0p = *c0p + c0;
+          __builtin_amdgcn_wave_barrier();
+          *c1p = *c1p + c1;
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+      continue;
+    }
     for (int pass = 0; pass < npass; ++pass) {
       BpSlot sl[S];
 #pragma unroll
