@@ -154,6 +154,69 @@ def batched_vs_plain(iters: int):
     }
 
 
+def parallel_cfg1(cpu: bool):
+    """Config 1: plain parallel-beam CGLS, 256 x 256 x 16 slices (1 mm),
+    180 angles over [0, 180 deg), 367 x 16 detector, 10-ellipsoid phantom
+    (seed 1, densities U[0.1, 1]), noise-free sinogram, 20 iterations from
+    x0 = 0; the CPU oracle runs the same 20 iterations."""
+    import numpy as np
+    import torch
+
+    from ccpi.reconstruction import workloads as W
+    from ccpi.reconstruction.projectors import cgls, forward_project, projector_nnz
+
+    g = W.parallel_geometry(256, 16, 180, 367)
+    ph = W.ellipsoid_phantom(g, 10, seed=1, dens=(0.1, 1.0))
+    b = forward_project(g, ph)
+    cgls(g, b, 1)                                # plan + warm kernels
+    torch.cuda.synchronize()
+    reps = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        x, _, _ = cgls(g, b, 20)
+        torch.cuda.synchronize()
+        reps.append(time.perf_counter() - t0)
+    secs = sorted(reps)[1]
+    rmse = float(torch.sqrt(torch.mean((x - ph) ** 2)))
+    out = {"workload": "parallel-beam CGLS 256x256x16, 180 angles, 367x16 detector, "
+                       "20 iterations, noise-free",
+           "iter_s": 20.0 / secs, "ms_per_iter": secs / 20 * 1e3,
+           "rmse_vs_phantom": rmse, "nnz": projector_nnz(g)}
+    if cpu:
+        import oracle
+        og = oracle.Geometry(g.angles, g.h_pixels, g.v_pixels, g.origin, g.voxel_size,
+                             g.shape)
+        bn = b.cpu().numpy()
+        t0 = time.perf_counter()
+        xo, _, _ = oracle.cgls(og, bn, 20)
+        tc = time.perf_counter() - t0
+        out["cpu_oracle_iter_s"] = 20.0 / tc
+        out["cpu_cores"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["max_abs_diff_vs_cpu_oracle"] = float(np.abs(x.cpu().numpy() - xo).max())
+    return out
+
+
+def cone_sirt_mlem(geom, b, iters: int):
+    """Config 2: cone-beam SIRT and MLEM at the 512^3 geometry, `iters`
+    iterations each on the headline sinogram; per-iteration time is
+    (T(iters) - T(1)) / (iters - 1), so the set-up projections are excluded."""
+    import torch
+
+    from ccpi.reconstruction.projectors import mlem, sirt
+    out = {}
+    for name, fn in (("sirt", sirt), ("mlem", mlem)):
+        ts = []
+        for n in (1, iters):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn(geom, b, n)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        out[f"{name}_iter_s"] = (iters - 1) / max(ts[1] - ts[0], 1e-9)
+    out["workload"] = f"cone-beam SIRT and MLEM 512^3, {iters} iterations each"
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +226,8 @@ def main() -> int:
                     help="angles in the CPU-oracle sample (0 disables it)")
     ap.add_argument("--batched-iters", type=int, default=20,
                     help="CGLS iterations of the batched-vs-plain comparison (0 skips it)")
+    ap.add_argument("--extras", type=int, default=1,
+                    help="also measure config 1 (parallel CGLS) and config 2 SIRT/MLEM")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device to device (the known-size kernel that "
                          "scales FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -199,7 +264,8 @@ def main() -> int:
     n_rays = int(np.prod(geom.sino_shape))
     bytes_alg = 4 * nnz + 4 * (n_rays + n_vox)
 
-    sess = CglsSession(geom, b, x, args.warmup + args.steps, timing=True)
+    # the session keeps its residual in the sinogram buffer it is given
+    sess = CglsSession(geom, b.clone(), x, args.warmup + args.steps, timing=True)
     log(f"bench: {args.warmup} warmup iterations")
     sess.iterate(args.warmup)
     torch.cuda.synchronize()
@@ -225,6 +291,12 @@ def main() -> int:
     if args.batched_iters > 0:
         log("bench: batched vs plain (64 parallel-beam problems)")
         batched = batched_vs_plain(args.batched_iters)
+
+    extras = None
+    if args.extras:
+        log("bench: config 1 (parallel CGLS) and config 2 SIRT / MLEM")
+        extras = {"cfg1": parallel_cfg1(cpu=args.cpu_sample > 0),
+                  "cfg2": cone_sirt_mlem(geom, b, 4)}
 
     base = None
     if args.cpu_sample > 0:
@@ -267,6 +339,7 @@ def main() -> int:
         "rmse_vs_phantom": rmse,
         "norm_atr_last": float(norms[-1]) if len(norms) else None,
         "batched": batched,
+        "extras": extras,
         "cpu_baseline": base,
     }
     print(json.dumps(line), flush=True)

@@ -726,20 +726,21 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
         d4[t] = lower ? -4 : 4;
         zb[t] = s_vz1 + (lower ? 0 : 1);
       }
-      for (int e = 0; e < C.n; ++e) {
-        const BpRec rec = R[e];
-        const int vlo = (int)(rec.w & 0xffffu), vhi = (int)(rec.w >> 16);
-        if (vb >= vhi || vb + span <= vlo) continue;  // block outside the live rows
-        const float a0 = __builtin_bit_cast(float, rec.y);
-        const float a1 = __builtin_bit_cast(float, rec.z);
-        const float ai = a0 * inv_dz;
+      auto live = [&](const BpRec &rec) {
+        return vb < (int)(rec.w >> 16) && vb + span > (int)(rec.w & 0xffffu);
+      };
+      auto gather = [&](const BpRec &rec, float *p) {
         const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
             (void *)(dpix + (size_t)rec.x * g.nvp), (short)0, nvp4, kRsrcWord3);
-        float p[S];
 #pragma unroll
         for (int t = 0; t < S; ++t)
           p[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
                                               prow, (int)lane_off, 4 * (vb + t), 0));
+      };
+      auto update = [&](const BpRec &rec, const float *p) {
+        const float a0 = __builtin_bit_cast(float, rec.y);
+        const float a1 = __builtin_bit_cast(float, rec.z);
+        const float ai = a0 * inv_dz;
 #pragma unroll
         for (int t = 0; t < S; ++t) {
           const int k = med3_i32((int)(g.pzbz + ai * dzv[t]), 0, nz - 1);
@@ -752,6 +753,26 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
           __builtin_amdgcn_wave_barrier();
           *c1p = *c1p + c1;
           __builtin_amdgcn_wave_barrier();
+        }
+      };
+      // records in pairs: both gathers are in flight before the first
+      // update (the updates stay in record order)
+      int e = 0;
+      for (; e + 1 < C.n; e += 2) {
+        const BpRec r0 = R[e], r1 = R[e + 1];
+        const bool l0 = live(r0), l1 = live(r1);
+        float p0[S], p1[S];
+        if (l0) gather(r0, p0);
+        if (l1) gather(r1, p1);
+        if (l0) update(r0, p0);
+        if (l1) update(r1, p1);
+      }
+      if (e < C.n) {
+        const BpRec r0 = R[e];
+        float p0[S];
+        if (live(r0)) {
+          gather(r0, p0);
+          update(r0, p0);
         }
       }
       continue;
