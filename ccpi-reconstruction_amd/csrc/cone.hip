@@ -432,6 +432,10 @@ __global__ __launch_bounds__(kFpRows) void cone_nnz_kernel(
 // ---------------------------------------------------------------------------
 
 constexpr int kBpWaves = 4;
+// records per step of the BP sweep (512^3 cone, S = 2: 2 -> 118 ms, 4 -> 111,
+// 8 -> 104, 12 -> 134 as register pressure cuts occupancy)
+template <int S>
+constexpr int bp_recs() { return S <= 2 ? 8 : 4; }
 
 // Detector columns covered by the four corner lines of one voxel column
 // (c2D.cpp:837-901).
@@ -755,19 +759,23 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
           __builtin_amdgcn_wave_barrier();
         }
       };
-      // records in pairs: both gathers are in flight before the first
-      // update (the updates stay in record order)
+      // kBpRecs records at a time: all their gathers are in flight before
+      // the first update (the updates stay in record order)
       int e = 0;
-      for (; e + 1 < C.n; e += 2) {
-        const BpRec r0 = R[e], r1 = R[e + 1];
-        const bool l0 = live(r0), l1 = live(r1);
-        float p0[S], p1[S];
-        if (l0) gather(r0, p0);
-        if (l1) gather(r1, p1);
-        if (l0) update(r0, p0);
-        if (l1) update(r1, p1);
+      constexpr int kBpRecs = bp_recs<S>();
+      for (; e + kBpRecs <= C.n; e += kBpRecs) {
+        BpRec r[kBpRecs];
+        float p[kBpRecs][S];
+#pragma unroll
+        for (int u = 0; u < kBpRecs; ++u) r[u] = R[e + u];
+#pragma unroll
+        for (int u = 0; u < kBpRecs; ++u)
+          if (live(r[u])) gather(r[u], p[u]);
+#pragma unroll
+        for (int u = 0; u < kBpRecs; ++u)
+          if (live(r[u])) update(r[u], p[u]);
       }
-      if (e < C.n) {
+      for (; e < C.n; ++e) {
         const BpRec r0 = R[e];
         float p0[S];
         if (live(r0)) {
@@ -781,20 +789,21 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
       BpSlot sl[S];
 #pragma unroll
       for (int t = 0; t < S; ++t) sl[t] = bp_slot(g, vb + S * lane + t, pass, one_set, min_v, max_v);
-      for (int e = 0; e < C.n; ++e) {
-        const BpRec rec = R[e];
-        const int vlo = (int)(rec.w & 0xffffu), vhi = (int)(rec.w >> 16);
-        if (vb >= vhi || vb + span <= vlo) continue;  // block outside the live rows
-        const float a0 = __builtin_bit_cast(float, rec.y);
-        const float a1 = __builtin_bit_cast(float, rec.z);
-        const float ai = a0 * inv_dz;
+      auto live = [&](const BpRec &rec) {
+        return vb < (int)(rec.w >> 16) && vb + span > (int)(rec.w & 0xffffu);
+      };
+      auto gather = [&](const BpRec &rec, float *p) {
         const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
             (void *)(dpix + (size_t)rec.x * g.nvp), (short)0, nvp4, kRsrcWord3);
-        float p[S];
 #pragma unroll
         for (int t = 0; t < S; ++t)
           p[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
                                               prow, (int)lane_off, 4 * (vb + t), 0));
+      };
+      auto update = [&](const BpRec &rec, const float *p) {
+        const float a0 = __builtin_bit_cast(float, rec.y);
+        const float a1 = __builtin_bit_cast(float, rec.z);
+        const float ai = a0 * inv_dz;
 #pragma unroll
         for (int t = 0; t < S; ++t) {
           const int k = (int)(sl[t].kb + ai * sl[t].dzv) - sl[t].ks;
@@ -815,6 +824,30 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
             ac[i1] = ac[i1] + c1;
             __builtin_amdgcn_wave_barrier();
           }
+        }
+      };
+      // kBpRecs records at a time: all their gathers are in flight before
+      // the first update (the updates stay in record order)
+      int e = 0;
+      constexpr int kBpRecs = bp_recs<S>();
+      for (; e + kBpRecs <= C.n; e += kBpRecs) {
+        BpRec r[kBpRecs];
+        float p[kBpRecs][S];
+#pragma unroll
+        for (int u = 0; u < kBpRecs; ++u) r[u] = R[e + u];
+#pragma unroll
+        for (int u = 0; u < kBpRecs; ++u)
+          if (live(r[u])) gather(r[u], p[u]);
+#pragma unroll
+        for (int u = 0; u < kBpRecs; ++u)
+          if (live(r[u])) update(r[u], p[u]);
+      }
+      for (; e < C.n; ++e) {
+        const BpRec r0 = R[e];
+        float p0[S];
+        if (live(r0)) {
+          gather(r0, p0);
+          update(r0, p0);
         }
       }
     }
