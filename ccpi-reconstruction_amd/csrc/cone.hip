@@ -291,7 +291,7 @@ __device__ inline __amdgpu_buffer_rsrc_t column_rsrc(const float *vol, uint32_t 
                                            kRsrcWord3);
 }
 
-constexpr int kFpGroup = 4;
+constexpr int kFpGroup = 8;
 
 // Segments [mb, me) of one ray for the wave's rows.  Groups of kFpGroup
 // segments issue all their gathers before the in-order accumulation (the
