@@ -5,7 +5,7 @@
 // the iteration body is cgls_3d, Core/src/Algorithms/cgls.cpp:26-157).
 // Every step of an iteration is ONE persistent launch over a flat work list
 // that spans all problems:
-//   FP    items (problem, 256-ray block, row block)   -> Ad, partial ||Ad||^2
+//   FP    items (problem, 256/lr-ray block, 4*lr-row block) -> Ad, partial ||Ad||^2
 //   alpha one workgroup per problem: fixed-order sum of its partials
 //   x/r   items (problem, element chunk)               -> x += a d, r -= a Ad
 //   BP    items (problem, 256-column block, slice block) -> s, partial ||s||^2
@@ -46,7 +46,7 @@ struct BatchProb {
   double gamma;
   float alpha, beta;
   int it;
-  int pad;
+  int fp_lr;           // FP lanes per ray
 };
 
 struct BatchLists {
@@ -55,8 +55,8 @@ struct BatchLists {
   const int *bp_off;
   const int *vx_off;   // volume chunks
   const int *px_off;   // sinogram chunks
-  double *part_fp;     // [fp items]
-  double *part_bp;     // [bp items]
+  double *part_fp;     // [4 * fp items]: one per wave item
+  double *part_bp;     // [4 * bp items]
 };
 
 constexpr int kChunk = 2048;  // elements per vector work item
@@ -70,39 +70,43 @@ __device__ inline int find_prob(const int *off, int count, int item) {
   return lo;
 }
 
-__device__ inline double block_sum_d(double v) {
-  __shared__ double sh[4];
+__device__ inline double wave_sum_d(double v) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-  __syncthreads();
-  const double t = sh[0] + sh[1] + sh[2] + sh[3];
-  __syncthreads();
-  return t;
+  return v;
 }
 
 // FP of d into Ad for every problem (mode 0), or BP of r into s (mode 1),
-// or BP of r into d (mode 2, initial direction).
+// or BP of r into d (mode 2, initial direction).  Work is taken per
+// wavefront: wave item w is quarter w % 4 (64 threads) of block item w / 4,
+// so no wave waits at a barrier for the longest walk of its workgroup; each
+// wave item leaves one FP64 partial norm.
 template <int RC, int MODE>
 __global__ __launch_bounds__(256) void batch_proj_kernel(BatchProb *__restrict__ P,
                                                          BatchLists L) {
   const int *off = MODE == 0 ? L.fp_off : L.bp_off;
   double *part = MODE == 0 ? L.part_fp : L.part_bp;
-  const int total = off[L.count];
-  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+  const int total = 4 * off[L.count];
+  const int lane = threadIdx.x & 63;
+  const int wid = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  for (int w = wid; w < total; w += 4 * gridDim.x) {
+    const int item = w >> 2;
+    const int tid = (w & 3) * 64 + lane;
     const int p = find_prob(off, L.count, item);
     const BatchProb &B = P[p];
+    // the problem's geometry as a local value (scalar registers), as the
+    // one-problem kernels get it through their arguments
+    const ParArgs g = B.g;
     const int local = item - off[p];
     double ss;
     if (MODE == 0) {
-      ss = par_fp_thread<RC>(B.g, local % B.fp_nbx, local / B.fp_nbx, threadIdx.x, B.d,
-                             B.Ad, false);
+      ss = par_fp_lanes(g, local % B.fp_nbx, local / B.fp_nbx, tid, B.fp_lr, B.d, B.Ad, false);
     } else {
-      ss = par_bp_thread<RC>(B.g, local % B.bp_nbx, local / B.bp_nbx, threadIdx.x, B.r,
+      ss = par_bp_thread<RC>(g, local % B.bp_nbx, local / B.bp_nbx, tid, B.r,
                              MODE == 1 ? B.s : B.d, false);
     }
-    ss = block_sum_d(ss);
-    if (threadIdx.x == 0) part[item] = ss;
+    ss = wave_sum_d(ss);
+    if (lane == 0) part[w] = ss;
   }
 }
 
@@ -116,7 +120,7 @@ __global__ __launch_bounds__(256) void batch_scalar_kernel(BatchProb *__restrict
   const double *part = STEP == 1 ? L.part_fp : L.part_bp;
   __shared__ double sh[256];
   double s = 0.0;
-  for (int q = off[p] + threadIdx.x; q < off[p + 1]; q += 256) s += part[q];
+  for (int q = 4 * off[p] + threadIdx.x; q < 4 * off[p + 1]; q += 256) s += part[q];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -180,14 +184,33 @@ struct Tuned {
 std::mutex g_tune_mu;
 std::map<std::string, Tuned> g_tuned;
 
+int cu_count();
+
+// The persistent grid is capped at the workgroups that are resident at
+// once: the work items are dealt out by a fixed stride, so a workgroup that
+// only starts when another one ends would run its share at the tail.
+template <class K>
+int blocks_per_cu(K kernel) {
+  int per_cu = 0;
+  CCPI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
+  return std::max(per_cu, 1);
+}
+
+template <int RC, int MODE>
+void launch_one(int grid, BatchProb *P, const BatchLists &L, hipStream_t st) {
+  static const int per_cu = blocks_per_cu(batch_proj_kernel<RC, MODE>);
+  const int g = std::max(1, std::min(grid, per_cu * cu_count()));
+  hipLaunchKernelGGL((batch_proj_kernel<RC, MODE>), dim3(g), dim3(256), 0, st, P, L);
+}
+
 template <int RC>
 void launch_proj(int mode, int grid, BatchProb *P, const BatchLists &L, hipStream_t st) {
   if (mode == 0)
-    hipLaunchKernelGGL((batch_proj_kernel<RC, 0>), dim3(grid), dim3(256), 0, st, P, L);
+    launch_one<RC, 0>(grid, P, L, st);
   else if (mode == 1)
-    hipLaunchKernelGGL((batch_proj_kernel<RC, 1>), dim3(grid), dim3(256), 0, st, P, L);
+    launch_one<RC, 1>(grid, P, L, st);
   else
-    hipLaunchKernelGGL((batch_proj_kernel<RC, 2>), dim3(grid), dim3(256), 0, st, P, L);
+    launch_one<RC, 2>(grid, P, L, st);
 }
 
 void proj(int rc, int mode, int grid, BatchProb *P, const BatchLists &L, hipStream_t st) {
@@ -215,8 +238,9 @@ void layout(BatchRun &R, int rc, hipStream_t st) {
   for (int p = 0; p < R.count; p++) {
     BatchProb &B = R.host[p];
     const ParArgs &g = B.g;
-    B.fp_nbx = (g.na * g.nh + 255) / 256;
-    B.fp_nby = (g.nv + rc - 1) / rc;
+    B.fp_lr = par_fp_lanes_per_ray(g.nv);
+    B.fp_nbx = (g.na * g.nh + 256 / B.fp_lr - 1) / (256 / B.fp_lr);
+    B.fp_nby = (g.nv + 4 * B.fp_lr - 1) / (4 * B.fp_lr);
     B.bp_nbx = (g.nx * g.ny + 255) / 256;
     B.bp_nby = (g.nz + rc - 1) / rc;
     off[p + 1] = off[p] + B.fp_nbx * B.fp_nby;
@@ -229,8 +253,8 @@ void layout(BatchRun &R, int rc, hipStream_t st) {
   R.fp_total = off[R.count];
   R.bp_total = off[2 * R.count + 1];
   R.offs.upload(off, st);
-  R.part_fp.alloc((size_t)std::max(R.fp_total, 1) * sizeof(double));
-  R.part_bp.alloc((size_t)std::max(R.bp_total, 1) * sizeof(double));
+  R.part_fp.alloc((size_t)std::max(4 * R.fp_total, 1) * sizeof(double));  // per wave item
+  R.part_bp.alloc((size_t)std::max(4 * R.bp_total, 1) * sizeof(double));
   int *o = R.offs.as<int>();
   R.L.count = R.count;
   R.L.fp_off = o;

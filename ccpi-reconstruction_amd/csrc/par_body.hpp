@@ -9,62 +9,14 @@
 
 namespace ccpi {
 
-// One thread of the FP: ray r = bx * 256 + tid, detector rows
-// [by * RC, by * RC + RC).  `acc_in` selects y += A x (reference f2D) or
-// y = A x.  Returns the sum of squares of the rows written.
-template <int RC>
-__device__ inline double par_fp_thread(const ParArgs &g, int bx, int by, int tid,
-                                       const float *__restrict__ vol,
-                                       float *__restrict__ sino, bool acc_in) {
-  static_assert(RC % 4 == 0, "rows are read in groups of four");
-  const int r = bx * 256 + tid;
-  if (r >= g.na * g.nh) return 0.0;
-  const int a = r / g.nh, h = r - a * g.nh;
-  const int v0 = by * RC;
-  const int nz = g.nz, nx = g.nx, ny = g.ny;
-  float *__restrict__ out = sino + (size_t)r * g.nv + v0;
-  const int nrow = min(RC, g.nv - v0);
-  float acc[RC];
-  int kmap[RC];
-#pragma unroll
-  for (int q = 0; q < RC; q++) {
-    acc[q] = (acc_in && q < nrow) ? out[q] : 0.f;
-    const int k = q < nrow ? g.map[v0 + q] : -1;
-    kmap[q] = (k >= 0 && k < nz) ? k : -1;
-  }
+// The detector line of ray (a, h) walked through the xy grid (p2D.cpp:
+// 215-458): add(x, y, l) for every crossed cell, l its xy length.
+template <class Add>
+__device__ inline void par_walk(const ParArgs &g, int a, int h, Add &&add) {
+  const int nx = g.nx, ny = g.ny;
   const double c = g.atab[5 * a], s = g.atab[5 * a + 1];
   const double p2x = c * g.D - s * g.h_pix[h];
   const double p2y = s * g.D + c * g.h_pix[h];
-
-  // one row per slice (map_type 1): the lane's rows are the contiguous
-  // slices v0 .. v0 + nval - 1 of each column, read 16 bytes at a time
-  const int nval = g.map_type == 1 ? max(0, min(nrow, nz - v0)) : 0;
-  auto add = [&](int x, int y, float l) {
-    const float *__restrict__ col = vol + ((size_t)x * ny + y) * nz;
-    if (g.map_type == 1) {
-      const float *__restrict__ c = col + v0;
-#pragma unroll
-      for (int q4 = 0; q4 < RC / 4; q4++) {
-        if (4 * q4 + 3 < nval) {
-          float4 f;
-          __builtin_memcpy(&f, c + 4 * q4, sizeof(f));
-          acc[4 * q4] = acc[4 * q4] + f.x * l;
-          acc[4 * q4 + 1] = acc[4 * q4 + 1] + f.y * l;
-          acc[4 * q4 + 2] = acc[4 * q4 + 2] + f.z * l;
-          acc[4 * q4 + 3] = acc[4 * q4 + 3] + f.w * l;
-        } else {
-#pragma unroll
-          for (int q = 4 * q4; q < 4 * q4 + 4; q++)
-            if (q < nval) acc[q] = acc[q] + c[q] * l;
-        }
-      }
-      return;
-    }
-#pragma unroll
-    for (int q = 0; q < RC; q++)
-      if (kmap[q] >= 0) acc[q] = acc[q] + col[kmap[q]] * l;
-  };
-
   if (fabs(c) < kEps) {
     if (fabs(s) >= kEps) {  // line parallel to y (p2D.cpp:233-260)
       const int i = (int)floor((p2x - g.ox) / g.dx);
@@ -140,6 +92,56 @@ __device__ inline double par_fp_thread(const ParArgs &g, int bx, int by, int tid
       }
     }
   }
+}
+
+// FP of detector rows [v0, v0 + RC) of ray r (clipped to nv): `acc_in`
+// selects y += A x (reference f2D) or y = A x.  Returns the sum of squares
+// of the rows written.
+template <int RC>
+__device__ inline double par_fp_rows(const ParArgs &g, int r, int v0,
+                                     const float *__restrict__ vol,
+                                     float *__restrict__ sino, bool acc_in) {
+  static_assert(RC % 4 == 0, "rows are read in groups of four");
+  const int a = r / g.nh, h = r - a * g.nh;
+  const int nz = g.nz, ny = g.ny;
+  float *__restrict__ out = sino + (size_t)r * g.nv + v0;
+  const int nrow = min(RC, g.nv - v0);
+  float acc[RC];
+  int kmap[RC];
+#pragma unroll
+  for (int q = 0; q < RC; q++) {
+    acc[q] = (acc_in && q < nrow) ? out[q] : 0.f;
+    const int k = q < nrow ? g.map[v0 + q] : -1;
+    kmap[q] = (k >= 0 && k < nz) ? k : -1;
+  }
+  // one row per slice (map_type 1): the rows are the contiguous slices
+  // v0 .. v0 + nval - 1 of each column, read 16 bytes at a time
+  const int nval = g.map_type == 1 ? max(0, min(nrow, nz - v0)) : 0;
+  par_walk(g, a, h, [&](int x, int y, float l) {
+    const float *__restrict__ col = vol + ((size_t)x * ny + y) * nz;
+    if (g.map_type == 1) {
+      const float *__restrict__ cv = col + v0;
+#pragma unroll
+      for (int q4 = 0; q4 < RC / 4; q4++) {
+        if (4 * q4 + 3 < nval) {
+          float4 f;
+          __builtin_memcpy(&f, cv + 4 * q4, sizeof(f));
+          acc[4 * q4] = acc[4 * q4] + f.x * l;
+          acc[4 * q4 + 1] = acc[4 * q4 + 1] + f.y * l;
+          acc[4 * q4 + 2] = acc[4 * q4 + 2] + f.z * l;
+          acc[4 * q4 + 3] = acc[4 * q4 + 3] + f.w * l;
+        } else {
+#pragma unroll
+          for (int q = 4 * q4; q < 4 * q4 + 4; q++)
+            if (q < nval) acc[q] = acc[q] + cv[q] * l;
+        }
+      }
+      return;
+    }
+#pragma unroll
+    for (int q = 0; q < RC; q++)
+      if (kmap[q] >= 0) acc[q] = acc[q] + col[kmap[q]] * l;
+  });
   double ss = 0.0;
 #pragma unroll
   for (int q = 0; q < RC; q++)
@@ -148,6 +150,39 @@ __device__ inline double par_fp_thread(const ParArgs &g, int bx, int by, int tid
       ss += (double)acc[q] * (double)acc[q];
     }
   return ss;
+}
+
+// One thread of the FP, one ray per lane: ray bx * 256 + tid, rows
+// [by * RC, by * RC + RC).
+template <int RC>
+__device__ inline double par_fp_thread(const ParArgs &g, int bx, int by, int tid,
+                                       const float *__restrict__ vol,
+                                       float *__restrict__ sino, bool acc_in) {
+  const int r = bx * 256 + tid;
+  if (r >= g.na * g.nh) return 0.0;
+  return par_fp_rows<RC>(g, r, by * RC, vol, sino, acc_in);
+}
+
+// FP, ray-shared layout: `lr` lanes (a power of two, at most 16) share one
+// ray, each taking 4 consecutive detector rows, so the lanes of a ray read
+// one contiguous run of every crossed column.  Block bx holds rays
+// [bx * 256 / lr, ...), row block by the rows [by * 4 lr, (by + 1) * 4 lr).
+__device__ inline double par_fp_lanes(const ParArgs &g, int bx, int by, int tid, int lr,
+                                      const float *__restrict__ vol,
+                                      float *__restrict__ sino, bool acc_in) {
+  const int r = bx * (256 / lr) + tid / lr;
+  if (r >= g.na * g.nh) return 0.0;
+  const int v0 = (by * lr + (tid & (lr - 1))) * 4;
+  if (v0 >= g.nv) return 0.0;
+  return par_fp_rows<4>(g, r, v0, vol, sino, acc_in);
+}
+
+// Lanes per ray of the FP for nv detector rows: enough for the rows of one
+// ray (4 per lane), at most 16.
+inline int par_fp_lanes_per_ray(int nv) {
+  int lr = 1;
+  while (lr < 16 && 4 * lr < nv) lr *= 2;
+  return lr;
 }
 
 // One thread of the BP: column bx * 256 + tid, slices [by * RC, by * RC +

@@ -11,12 +11,13 @@
 // reference's 32 x 32 tiles, so only the float summation order differs;
 // the BP visits angles and columns in the reference's order.
 //
-// Mapping to CDNA4: one ray per lane.  A wavefront holds 64 adjacent
-// detector columns of one angle (neighbouring rays cross neighbouring
-// cells, so the per-lane voxel reads share cache lines), each lane keeps a
-// block of kRows detector rows in registers.  The BP mirrors it: one voxel
-// column per lane, kRows slices in registers, pixel rows read once per
-// (angle, h) hit.
+// Mapping to CDNA4.  FP: `lr` lanes share one ray (lr = nv / 4 rounded up
+// to a power of two, at most 16), each lane walks the ray and keeps 4
+// consecutive detector rows in registers, so the lanes of a ray read one
+// contiguous run of each crossed column (16 bytes per lane) and a wave
+// covers 64 / lr adjacent rays of one angle.  BP: one voxel column per
+// lane, kRows slices in registers, pixel rows read once per (angle, h)
+// hit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,10 +32,10 @@ namespace ccpi {
 
 constexpr int kRows = 16;
 
-__global__ __launch_bounds__(256) void par_fp_kernel(ParArgs g,
+__global__ __launch_bounds__(256) void par_fp_kernel(ParArgs g, int lr,
                                                      const float *__restrict__ vol,
                                                      float *__restrict__ sino) {
-  par_fp_thread<kRows>(g, blockIdx.x, blockIdx.y, threadIdx.x, vol, sino, true);
+  par_fp_lanes(g, blockIdx.x, blockIdx.y, threadIdx.x, lr, vol, sino, true);
 }
 
 __global__ __launch_bounds__(256) void par_bp_kernel(ParArgs g,
@@ -179,8 +180,9 @@ void par_build_plan(ParPlan &P, hipStream_t st) {
 void par_fp(const ParPlan &P, const float *vol, float *sino, hipStream_t st) {
   const ParArgs &A = P.args;
   const int rays = A.na * A.nh;
-  dim3 grid((unsigned)((rays + 255) / 256), (unsigned)((A.nv + kRows - 1) / kRows));
-  hipLaunchKernelGGL(par_fp_kernel, grid, dim3(256), 0, st, A, vol, sino);
+  const int lr = par_fp_lanes_per_ray(A.nv), rpb = 256 / lr;
+  dim3 grid((unsigned)((rays + rpb - 1) / rpb), (unsigned)((A.nv + 4 * lr - 1) / (4 * lr)));
+  hipLaunchKernelGGL(par_fp_kernel, grid, dim3(256), 0, st, A, lr, vol, sino);
   CCPI_HIP(hipGetLastError());
 }
 
