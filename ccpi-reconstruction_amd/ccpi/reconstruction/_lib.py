@@ -17,13 +17,15 @@ CCPI_PARALLEL = 0
 CCPI_CONE = 1
 CCPI_REDUCE_FP32 = 1
 CCPI_TIMING = 2
+CCPI_REG_TIKHONOV = 0
+CCPI_REG_TV = 1
 
 # every symbol include/ccpi_hip.h declares
 EXPORTS = (
     "ccpi_pb_f2D", "ccpi_pb_b2D", "ccpi_cb_f2D", "ccpi_cb_b2D",
     "ccpi_forward_project", "ccpi_backward_project",
-    "ccpi_cgls", "ccpi_sirt", "ccpi_mlem", "ccpi_pb_cgls_batched",
-    "ccpi_cgls_create", "ccpi_cgls_iterate", "ccpi_cgls_norms", "ccpi_cgls_timings",
+    "ccpi_cgls", "ccpi_sirt", "ccpi_mlem", "ccpi_regularize", "ccpi_cgls_reg",
+    "ccpi_pb_cgls_batched", "ccpi_cgls_create", "ccpi_cgls_iterate", "ccpi_cgls_norms", "ccpi_cgls_timings",
     "ccpi_cgls_destroy", "ccpi_projector_nnz",
     "ccpi_error_string", "ccpi_last_error", "ccpi_version",
     "ccpi_plan_cache_clear", "ccpi_plan_cache_bytes",
@@ -77,6 +79,8 @@ def lib():
     L.ccpi_cgls.argtypes = [gp, fp, fp, i, dp, i, vp]
     L.ccpi_sirt.argtypes = [gp, fp, fp, i, i, vp]
     L.ccpi_mlem.argtypes = [gp, fp, fp, i, i, vp]
+    L.ccpi_regularize.argtypes = [i, fp, fp, i, i, i, vp]
+    L.ccpi_cgls_reg.argtypes = [gp, fp, fp, i, i, d, dp, i, vp]
     L.ccpi_pb_cgls_batched.argtypes = [gp, i, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                        i, dp, i, vp]
     ip = C.POINTER(C.c_int)
@@ -88,7 +92,8 @@ def lib():
     L.ccpi_projector_nnz.argtypes = [gp, C.POINTER(C.c_ulonglong), vp]
     for name in ("ccpi_pb_f2D", "ccpi_pb_b2D", "ccpi_cb_f2D", "ccpi_cb_b2D",
                  "ccpi_forward_project", "ccpi_backward_project", "ccpi_cgls",
-                 "ccpi_sirt", "ccpi_mlem", "ccpi_pb_cgls_batched", "ccpi_cgls_create",
+                 "ccpi_sirt", "ccpi_mlem", "ccpi_regularize", "ccpi_cgls_reg",
+                 "ccpi_pb_cgls_batched", "ccpi_cgls_create",
                  "ccpi_cgls_iterate", "ccpi_cgls_norms", "ccpi_cgls_timings",
                  "ccpi_cgls_destroy", "ccpi_projector_nnz"):
         getattr(L, name).restype = C.c_int

@@ -40,7 +40,7 @@ def _device():
 def _reconstruct(algo, pixels, angles, h_offsets, v_offsets, pixels_per_voxel,
                  source_to_sample, source_to_detector, pixel_h_size, pixel_v_size,
                  mask_radius, beam_harden, full_vox_origin, voxel_size, niterations,
-                 nthreads, is_pixels_in_log):
+                 nthreads, is_pixels_in_log, lam=0.0):
     import torch
     dev = _device()
     pix = np.ascontiguousarray(np.asarray(pixels, dtype=np.float32))
@@ -56,6 +56,10 @@ def _reconstruct(algo, pixels, angles, h_offsets, v_offsets, pixels_per_voxel,
     x = torch.zeros(g.shape, dtype=torch.float32, device=dev)
     if algo == "cgls":
         x, _, norms = _pj.cgls(g, b, int(niterations), x)
+        _last["norms"] = norms
+    elif algo in ("tikhonov", "tv"):
+        kind = _pj.TIKHONOV if algo == "tikhonov" else _pj.TV
+        x, _, norms = _pj.cgls_reg(g, b, int(niterations), kind, float(lam), x)
         _last["norms"] = norms
     elif algo == "sirt":
         x = _pj.sirt(g, b, int(niterations), x)
@@ -96,6 +100,54 @@ def mlem(pixels, angles, h_offsets, v_offsets, pixels_per_voxel, source_to_sampl
                         nthreads, is_pixels_in_log)
 
 
+def _norm_quirk(norm_r, niterations):
+    """The reference zeroes norm_r[:niterations] when norm_r tests nonzero
+    and otherwise leaves it alone; it never stores the residual norms
+    (conebeam_wrapper.cpp:84-93).  A norm_r longer than one element makes
+    the reference's truth test raise; here its first element decides."""
+    if isinstance(norm_r, np.ndarray) and norm_r.size and norm_r.flat[0] != 0:
+        norm_r.reshape(-1)[:int(niterations)] = 0
+
+
+def cgls_conv(pixels, angles, h_offsets, v_offsets, pixels_per_voxel, source_to_sample,
+              source_to_detector, pixel_h_size, pixel_v_size, mask_radius, beam_harden,
+              full_vox_origin, voxel_size, niterations, nthreads, norm_r,
+              is_pixels_in_log=True):
+    """CGLS with a norm_r argument (conebeam_wrapper.cpp:215-231)."""
+    out = _reconstruct("cgls", pixels, angles, h_offsets, v_offsets, pixels_per_voxel,
+                       source_to_sample, source_to_detector, pixel_h_size, pixel_v_size,
+                       mask_radius, beam_harden, full_vox_origin, voxel_size, niterations,
+                       nthreads, is_pixels_in_log)
+    _norm_quirk(norm_r, niterations)
+    return out
+
+
+def cgls_tikhonov(pixels, angles, h_offsets, v_offsets, pixels_per_voxel, source_to_sample,
+                  source_to_detector, pixel_h_size, pixel_v_size, mask_radius, beam_harden,
+                  full_vox_origin, voxel_size, niterations, nthreads, regularize, norm_r,
+                  is_pixels_in_log=True):
+    """CGLS with a Tikhonov term (conebeam_wrapper.cpp:172-189)."""
+    out = _reconstruct("tikhonov", pixels, angles, h_offsets, v_offsets, pixels_per_voxel,
+                       source_to_sample, source_to_detector, pixel_h_size, pixel_v_size,
+                       mask_radius, beam_harden, full_vox_origin, voxel_size, niterations,
+                       nthreads, is_pixels_in_log, regularize)
+    _norm_quirk(norm_r, niterations)
+    return out
+
+
+def cgls_TVreg(pixels, angles, h_offsets, v_offsets, pixels_per_voxel, source_to_sample,
+               source_to_detector, pixel_h_size, pixel_v_size, mask_radius, beam_harden,
+               full_vox_origin, voxel_size, niterations, nthreads, regularize, norm_r,
+               is_pixels_in_log=True):
+    """CGLS with a TV term (conebeam_wrapper.cpp:191-213)."""
+    out = _reconstruct("tv", pixels, angles, h_offsets, v_offsets, pixels_per_voxel,
+                       source_to_sample, source_to_detector, pixel_h_size, pixel_v_size,
+                       mask_radius, beam_harden, full_vox_origin, voxel_size, niterations,
+                       nthreads, is_pixels_in_log, regularize)
+    _norm_quirk(norm_r, niterations)
+    return out
+
+
 def last_geometry():
     """Geometry (with the find_centre shift) of the last reconstruction."""
     return _last.get("geometry")
@@ -121,5 +173,6 @@ def create_sinogram(source_x, source_y, source_z, detector_x, h_pixels, v_pixels
     return _pj.forward_project(g, torch.from_numpy(vol).to(dev)).cpu().numpy()
 
 
-for _f in (cgls, sirt, mlem, last_geometry, cgls_norms, create_sinogram):
+for _f in (cgls, cgls_conv, sirt, mlem, cgls_tikhonov, cgls_TVreg, last_geometry, cgls_norms,
+           create_sinogram):
     setattr(alg, _f.__name__, _f)

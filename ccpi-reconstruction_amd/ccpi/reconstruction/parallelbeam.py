@@ -44,8 +44,18 @@ def _as_f32(a, name):
     return np.ascontiguousarray(a)
 
 
+def _norm_quirk(norm_r, niterations):
+    """The reference never writes residual norms into ``norm_r``: when its
+    first element is nonzero it overwrites the first ``niterations``
+    entries with zeros, otherwise it leaves the array alone
+    (diamond_wrapper.cpp:95-104, 213-222).  Kept as is; the norms are
+    available from ``alg.cgls_norms()``."""
+    if isinstance(norm_r, np.ndarray) and norm_r.size and norm_r.flat[0] != 0:
+        norm_r.reshape(-1)[:int(niterations)] = 0
+
+
 def _reconstruct(pixels, angles, rotation_centre, resolution, niterations, is_log, algo,
-                 x0=None):
+                 x0=None, lam=0.0):
     import torch
     dev = _device()
     pix = _as_f32(pixels, "pixels")
@@ -62,6 +72,10 @@ def _reconstruct(pixels, angles, rotation_centre, resolution, niterations, is_lo
         x.copy_(torch.from_numpy(x0))
     if algo == "cgls":
         x, _, norms = _pj.cgls(g, b, int(niterations), x)
+        _last["norms"] = norms
+    elif algo in ("tikhonov", "tv"):
+        kind = _pj.TIKHONOV if algo == "tikhonov" else _pj.TV
+        x, _, norms = _pj.cgls_reg(g, b, int(niterations), kind, float(lam), x)
         _last["norms"] = norms
     elif algo == "sirt":
         x = _pj.sirt(g, b, int(niterations), x)
@@ -114,8 +128,63 @@ def mlem_step(pixels, angles, rotation_centre, resolution, niterations, nthreads
                         is_pixels_in_log, "mlem", last_iteration_volume)
 
 
+def cgls_conv(pixels, angles, rotation_centre, resolution, niterations, nthreads, norm_r,
+              is_pixels_in_log=True):
+    """CGLS with a norm_r argument (diamond_wrapper.cpp:388-396)."""
+    out = _reconstruct(pixels, angles, rotation_centre, resolution, niterations,
+                       is_pixels_in_log, "cgls")
+    _norm_quirk(norm_r, niterations)
+    return out
+
+
+def cgls_conv_step(pixels, angles, rotation_centre, resolution, niterations, nthreads, norm_r,
+                   is_pixels_in_log, last_iteration_volume):
+    """cgls_conv from a previous volume (diamond_wrapper.cpp:398-409)."""
+    out = _reconstruct(pixels, angles, rotation_centre, resolution, niterations,
+                       is_pixels_in_log, "cgls", last_iteration_volume)
+    _norm_quirk(norm_r, niterations)
+    return out
+
+
+def cgls_tikhonov(pixels, angles, rotation_centre, resolution, niterations, nthreads,
+                  regularize, norm_r, is_pixels_in_log=True):
+    """CGLS with a Tikhonov term (diamond_wrapper.cpp:329-339, cgls.cpp:
+    343-430)."""
+    out = _reconstruct(pixels, angles, rotation_centre, resolution, niterations,
+                       is_pixels_in_log, "tikhonov", lam=regularize)
+    _norm_quirk(norm_r, niterations)
+    return out
+
+
+def cgls_tikhonov_step(pixels, angles, rotation_centre, resolution, niterations, nthreads,
+                       regularize, norm_r, is_pixels_in_log, last_iteration_volume):
+    """cgls_tikhonov from a previous volume (diamond_wrapper.cpp:341-352)."""
+    out = _reconstruct(pixels, angles, rotation_centre, resolution, niterations,
+                       is_pixels_in_log, "tikhonov", last_iteration_volume, regularize)
+    _norm_quirk(norm_r, niterations)
+    return out
+
+
+def cgls_TVreg(pixels, angles, rotation_centre, resolution, niterations, nthreads, regularize,
+               norm_r, is_pixels_in_log=True):
+    """CGLS with a TV term (diamond_wrapper.cpp:358-368, cgls.cpp:343-435)."""
+    out = _reconstruct(pixels, angles, rotation_centre, resolution, niterations,
+                       is_pixels_in_log, "tv", lam=regularize)
+    _norm_quirk(norm_r, niterations)
+    return out
+
+
+def cgls_TVreg_step(pixels, angles, rotation_centre, resolution, niterations, nthreads,
+                    regularize, norm_r, is_pixels_in_log, last_iteration_volume):
+    """cgls_TVreg from a previous volume (diamond_wrapper.cpp:370-381)."""
+    out = _reconstruct(pixels, angles, rotation_centre, resolution, niterations,
+                       is_pixels_in_log, "tv", last_iteration_volume, regularize)
+    _norm_quirk(norm_r, niterations)
+    return out
+
+
 def cgls_norms():
-    """||A^T r|| per iteration of the last cgls / cgls_step call."""
+    """||A^T r|| per iteration of the last CGLS-family call."""
     return np.array(_last.get("norms", np.zeros(0)))
 
 
@@ -171,7 +240,8 @@ def pb_backward_project(projections, angles, rotation_centre, resolution):
     return _pj.backward_project(g, s.contiguous()).cpu().numpy()
 
 
-for _f in (cgls, sirt, mlem, cgls_step, sirt_step, mlem_step, cgls_norms,
+for _f in (cgls, sirt, mlem, cgls_step, sirt_step, mlem_step, cgls_conv, cgls_conv_step,
+           cgls_tikhonov, cgls_tikhonov_step, cgls_TVreg, cgls_TVreg_step, cgls_norms,
            pb_setup_geometry_from_image, pb_setup_geometry_from_acquisition,
            pb_forward_project, pb_backward_project):
     setattr(alg, _f.__name__, _f)

@@ -17,8 +17,8 @@ from typing import Sequence
 
 import numpy as np
 
-from ._lib import (CCPI_CONE, CCPI_PARALLEL, CCPI_REDUCE_FP32, CCPI_TIMING, CcpiGeometry,
-                   check, lib)
+from ._lib import (CCPI_CONE, CCPI_PARALLEL, CCPI_REDUCE_FP32, CCPI_REG_TIKHONOV,
+                   CCPI_REG_TV, CCPI_TIMING, CcpiGeometry, check, lib)
 
 
 def _f64(a) -> np.ndarray:
@@ -130,6 +130,40 @@ def cgls(geom: Geometry, b, iterations: int, x=None, reduce_fp32: bool = False):
                           _dev(x, geom.shape, "x"), int(iterations),
                           norms.ctypes.data_as(C.POINTER(C.c_double)),
                           CCPI_REDUCE_FP32 if reduce_fp32 else 0, _stream()))
+    return x, r, norms[:iterations]
+
+
+TIKHONOV, TV = CCPI_REG_TIKHONOV, CCPI_REG_TV
+
+
+def regularize(kind: int, vol, out=None):
+    """Tikhonov (``TIKHONOV``, tikhonov.cpp:4-32) or TV (``TV``,
+    total_v.cpp:21-195) term of a [nx][ny][nz] float32 device volume; every
+    extent must be >= 2."""
+    import torch
+    if vol.dim() != 3:
+        raise ValueError("vol must be [nx][ny][nz]")
+    shape = tuple(vol.shape)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=vol.device)
+    check(lib().ccpi_regularize(int(kind), _dev(vol, shape, "vol"), _dev(out, shape, "out"),
+                                shape[0], shape[1], shape[2], _stream()))
+    return out
+
+
+def cgls_reg(geom: Geometry, b, iterations: int, kind: int, lam: float, x=None,
+             reduce_fp32: bool = False):
+    """Regularised CGLS, cgls_tikhonov / cgls_tv_reg (cgls.cpp:343-435).
+    Returns (x, residual, norms) like :func:`cgls`; ``b`` is not modified."""
+    import torch
+    r = b.detach().clone().contiguous()
+    if x is None:
+        x = torch.zeros(geom.shape, dtype=torch.float32, device=b.device)
+    norms = np.zeros(max(iterations, 1), np.float64)
+    check(lib().ccpi_cgls_reg(C.byref(geom.c_struct()), _dev(r, geom.sino_shape, "b"),
+                              _dev(x, geom.shape, "x"), int(iterations), int(kind),
+                              float(lam), norms.ctypes.data_as(C.POINTER(C.c_double)),
+                              CCPI_REDUCE_FP32 if reduce_fp32 else 0, _stream()))
     return x, r, norms[:iterations]
 
 

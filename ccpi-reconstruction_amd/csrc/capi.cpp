@@ -125,6 +125,36 @@ int ccpi_cgls(const ccpi_geometry *g, float *b, float *x, int iterations, double
   });
 }
 
+int ccpi_regularize(int kind, const float *a, float *out, int nx, int ny, int nz,
+                    void *stream) {
+  return guarded([&] {
+    CCPI_REQUIRE(a && out, CCPI_E_ARG, "null data pointer");
+    CCPI_REQUIRE(a != out, CCPI_E_ARG, "in-place regularisation is not supported");
+    CCPI_REQUIRE(kind == CCPI_REG_TIKHONOV || kind == CCPI_REG_TV, CCPI_E_ARG,
+                 "unknown regularisation kind");
+    ccpi::check_reg_dims(nx, ny, nz);
+    ccpi::DeviceBuffer scr;
+    const size_t nscr = ccpi::reg_scratch_floats(kind, (size_t)nx * ny * nz);
+    if (nscr) scr.alloc(nscr * sizeof(float));
+    ccpi::reg_term(kind, a, out, nscr ? scr.as<float>() : nullptr, nx, ny, nz, S(stream));
+    CCPI_HIP(hipStreamSynchronize(S(stream)));
+  });
+}
+
+int ccpi_cgls_reg(const ccpi_geometry *g, float *b, float *x, int iterations, int kind,
+                  double lambda, double *norms, int flags, void *stream) {
+  return guarded([&] {
+    CCPI_REQUIRE(b && x, CCPI_E_ARG, "null data pointer");
+    CCPI_REQUIRE(iterations >= 0, CCPI_E_ARG, "negative iteration count");
+    CCPI_REQUIRE(kind == CCPI_REG_TIKHONOV || kind == CCPI_REG_TV, CCPI_E_ARG,
+                 "unknown regularisation kind");
+    ccpi::Projector P(ccpi::Geometry::from(g), S(stream));
+    ccpi::Workspace W;
+    ccpi::run_cgls_reg(P, b, x, iterations, kind, lambda, norms,
+                       (flags & CCPI_REDUCE_FP32) != 0, W, S(stream));
+  });
+}
+
 int ccpi_sirt(const ccpi_geometry *g, const float *b, float *x, int iterations, int flags,
               void *stream) {
   (void)flags;

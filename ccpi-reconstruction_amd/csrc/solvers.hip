@@ -125,6 +125,54 @@ __global__ __launch_bounds__(kRedThreads) void cgls_beta_kernel(const Acc *part,
   }
 }
 
+// alpha = gamma / (||Ad||^2 + ||Ad_vox||^2) (cgls.cpp:374-377); in float
+// mode the two float norms are added in float as the reference does
+template <class Acc>
+__global__ __launch_bounds__(kRedThreads) void cgls_alpha_reg_kernel(const Acc *part_pix,
+                                                                     const Acc *part_vox,
+                                                                     CglsScalars *sc,
+                                                                     int fp32) {
+  const Acc sp = sum_partials(part_pix);
+  __syncthreads();
+  const Acc sv = sum_partials(part_vox);
+  if (threadIdx.x == 0) {
+    if (fp32) {
+      const float den = (float)sp + (float)sv;
+      sc->alpha_f = sc->gamma_f / den;
+    } else {
+      const double den = (double)sp + (double)sv;
+      sc->alpha_f = den > 0.0 ? (float)(sc->gamma / den) : 0.f;
+    }
+  }
+}
+
+// y += a x, with per-workgroup partial sums of the new y^2 (sum_axpy then
+// norm_voxels, cgls.cpp:373-376)
+template <class Acc>
+__global__ __launch_bounds__(kRedThreads) void axpy_sumsq_kernel(float *__restrict__ y,
+                                                                 const float *__restrict__ x,
+                                                                 size_t n, float a,
+                                                                 Acc *__restrict__ part) {
+  Acc s = 0;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+    const float v = y[q] + a * x[q];
+    y[q] = v;
+    const Acc w = (Acc)v;
+    s += w * w;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// y += a x (sum_axpy, blas.hpp:307-322)
+__global__ void axpy_kernel(float *__restrict__ y, const float *__restrict__ x, size_t n,
+                            float a) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+    y[q] = y[q] + a * x[q];
+}
+
 // x += alpha d (voxels) and r -= alpha Ad (pixels) (cgls.cpp:120-121)
 __global__ void cgls_update_xr_kernel(float *__restrict__ x, const float *__restrict__ d,
                                       size_t nvox, float *__restrict__ r,
@@ -324,6 +372,84 @@ void run_cgls(const Projector &P, float *b, float *x, int iters, double *norms, 
   run.start(st);
   for (int it = 0; it < iters; it++) run.iterate(it, st);
   run.finish(norms, st);
+}
+
+// cgls_regularize::reconstruct (cgls.cpp:343-423).  Per iteration:
+// Ad = A d; Ad_vox += lambda L(d) (never reset); alpha = gamma / (||Ad||^2 +
+// ||Ad_vox||^2); x += alpha d; r -= alpha Ad; x += lambda L(x); s = A^T r;
+// beta, d = s + beta d.  The reference's update of an unused voxel residual
+// (cgls.cpp:390, indexed with the sinogram's extents) has no effect on x or
+// the norms and is not performed.
+void run_cgls_reg(const Projector &P, float *b, float *x, int iters, int kind, double lambda,
+                  double *norms, bool fp32, Workspace &W, hipStream_t st) {
+  const Geometry &G = P.geometry();
+  check_reg_dims(G.nx, G.ny, G.nz);
+  const size_t nvox = G.n_vox(), npix = G.n_pix();
+  W.ensure(nvox, npix);
+  W.v4.alloc(nvox * 4);
+  W.part2.alloc(kRedBlocks * sizeof(double));
+  const size_t nscr = reg_scratch_floats(kind, nvox);
+  if (nscr) W.tvd.alloc(nscr * 4);
+  DeviceBuffer norms_dev;
+  norms_dev.alloc((size_t)std::max(iters, 1) * sizeof(double));
+  float *d = W.v1.as<float>(), *s = W.v2.as<float>(), *L = W.v3.as<float>();
+  float *adv = W.v4.as<float>(), *Ad = W.p1.as<float>();
+  float *scr = nscr ? W.tvd.as<float>() : nullptr;
+  CglsScalars *sc = W.scal.as<CglsScalars>();
+  const float lam = (float)lambda;
+  const int gv = grid_for(nvox);
+  dev_fill(d, nvox, 0.f, st);
+  dev_fill(adv, nvox, 0.f, st);
+  P.bp(b, d, st);
+  if (fp32) {
+    sumsq<float>(d, nvox, W.part.p, st);
+    hipLaunchKernelGGL(cgls_init_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
+                       W.part.as<float>(), sc);
+  } else {
+    sumsq<double>(d, nvox, W.part.p, st);
+    hipLaunchKernelGGL(cgls_init_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
+                       W.part.as<double>(), sc);
+  }
+  for (int it = 0; it < iters; it++) {
+    dev_fill(Ad, npix, 0.f, st);
+    P.fp(d, Ad, st);
+    reg_term(kind, d, L, scr, G.nx, G.ny, G.nz, st);
+    if (fp32) {
+      sumsq<float>(Ad, npix, W.part.p, st);
+      hipLaunchKernelGGL(axpy_sumsq_kernel<float>, dim3(kRedBlocks), dim3(kRedThreads), 0, st,
+                         adv, L, nvox, lam, W.part2.as<float>());
+      hipLaunchKernelGGL(cgls_alpha_reg_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
+                         W.part.as<float>(), W.part2.as<float>(), sc, 1);
+    } else {
+      sumsq<double>(Ad, npix, W.part.p, st);
+      hipLaunchKernelGGL(axpy_sumsq_kernel<double>, dim3(kRedBlocks), dim3(kRedThreads), 0,
+                         st, adv, L, nvox, lam, W.part2.as<double>());
+      hipLaunchKernelGGL(cgls_alpha_reg_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
+                         W.part.as<double>(), W.part2.as<double>(), sc, 0);
+    }
+    hipLaunchKernelGGL(cgls_update_xr_kernel, dim3(grid_for(std::max(nvox, npix))), dim3(256),
+                       0, st, x, d, nvox, b, Ad, npix, sc);
+    reg_term(kind, x, L, scr, G.nx, G.ny, G.nz, st);
+    hipLaunchKernelGGL(axpy_kernel, dim3(gv), dim3(256), 0, st, x, L, nvox, lam);
+    dev_fill(s, nvox, 0.f, st);
+    P.bp(b, s, st);
+    if (fp32) {
+      sumsq<float>(s, nvox, W.part.p, st);
+      hipLaunchKernelGGL(cgls_beta_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
+                         W.part.as<float>(), sc, norms_dev.as<double>(), it, 1);
+    } else {
+      sumsq<double>(s, nvox, W.part.p, st);
+      hipLaunchKernelGGL(cgls_beta_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
+                         W.part.as<double>(), sc, norms_dev.as<double>(), it, 0);
+    }
+    hipLaunchKernelGGL(cgls_update_d_kernel, dim3(gv), dim3(256), 0, st, d, s, nvox, sc);
+    CCPI_HIP(hipGetLastError());
+  }
+  if (norms && iters > 0) {
+    CCPI_HIP(hipMemcpyAsync(norms, norms_dev.p, (size_t)iters * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+  }
+  CCPI_HIP(hipStreamSynchronize(st));
 }
 
 void run_sirt(const Projector &P, const float *b, float *x, int iters, Workspace &W,

@@ -22,9 +22,20 @@ struct Workspace {
   DeviceBuffer p1, p2;       // sinogram-sized
   DeviceBuffer part;         // reduction partials
   DeviceBuffer scal;         // CglsScalars
+  DeviceBuffer v4, part2;    // regularised CGLS: Ad_vox, its partials
+  DeviceBuffer tvd;          // TV term: D1, D2, D3
   ~Workspace();
   void ensure(size_t nvox, size_t npix);
 };
+
+// regularize.hip: Tikhonov / TV terms (tikhonov.cpp, total_v.cpp)
+constexpr int kRegTikhonov = 0;
+constexpr int kRegTV = 1;
+void check_reg_dims(int nx, int ny, int nz);
+size_t reg_scratch_floats(int kind, size_t nvox);
+// out = L(a); scratch holds reg_scratch_floats(kind, nvox) floats
+void reg_term(int kind, const float *a, float *out, float *scratch, int nx, int ny, int nz,
+              hipStream_t st);
 
 // HIP events bracketing projector launches (resolved on demand).
 struct EventLog {
@@ -53,6 +64,9 @@ struct CglsRun {
 void dev_fill(float *p, size_t n, float v, hipStream_t st);
 void run_cgls(const Projector &P, float *b, float *x, int iters, double *norms,
               bool fp32, Workspace &W, hipStream_t st);
+// regularised CGLS (cgls.cpp:343-435), kind kRegTikhonov or kRegTV
+void run_cgls_reg(const Projector &P, float *b, float *x, int iters, int kind, double lambda,
+                  double *norms, bool fp32, Workspace &W, hipStream_t st);
 void run_sirt(const Projector &P, const float *b, float *x, int iters,
               Workspace &W, hipStream_t st);
 void run_mlem(const Projector &P, const float *b, float *x, int iters,

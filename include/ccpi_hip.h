@@ -115,6 +115,25 @@ int ccpi_sirt(const ccpi_geometry *g, const float *b, float *x,
 int ccpi_mlem(const ccpi_geometry *g, const float *b, float *x,
               int iterations, int flags, void *stream);
 
+/* ---- regularised CGLS ---------------------------------------------------
+   Regularisation terms CCPi::tikhonov_regularize / CCPi::tv_regularize
+   (Core/src/regularize.hpp:6-9; tikhonov.cpp:4-32, total_v.cpp:21-195):
+   out = L(a) for a [nx][ny][nz] device volume.  Neighbours past a face are
+   mirrored, so every extent must be >= 2 (CCPI_E_ARG otherwise).  a and
+   out must not overlap. */
+#define CCPI_REG_TIKHONOV    0
+#define CCPI_REG_TV          1
+int ccpi_regularize(int kind, const float *a, float *out, int nx, int ny,
+                    int nz, void *stream);
+
+/* cgls_tikhonov / cgls_tv_reg (Core/src/Algorithms/cgls.hpp:103-131,
+   cgls.cpp:343-435) with regularisation parameter lambda.  Same buffer
+   contract as ccpi_cgls: b becomes the residual, x receives x + the
+   iterations' updates, norms[i] = ||A^T r|| after iteration i. */
+int ccpi_cgls_reg(const ccpi_geometry *g, float *b, float *x, int iterations,
+                  int kind, double lambda, double *norms, int flags,
+                  void *stream);
+
 /* ---- CGLS session: the same body split into launch-only steps ----------
    No counterpart in the reference (its loop is internal to
    cgls_base::reconstruct, cgls.cpp:54-82).  For callers that interleave

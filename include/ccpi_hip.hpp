@@ -148,6 +148,42 @@ class cgls_3d : public reconstruction_alg {
   std::vector<double> norms_;
 };
 
+// cgls_regularize / cgls_tikhonov / cgls_tv_reg (cgls.hpp:103-131,
+// cgls.cpp:343-435): the CGLS loop with lambda L(.) added to the normal
+// equations' voxel term and to x each iteration.
+class cgls_regularize : public reconstruction_alg {
+ public:
+  cgls_regularize(int iterations, double param, int kind, int flags = 0)
+      : reconstruction_alg(iterations, flags), param_(param), kind_(kind) {}
+  bool reconstruct(const instrument &device, float *pixels, float *voxels,
+                   const double origin[3], const double voxel_size[3], int nx, int ny,
+                   int nz, void *stream = nullptr) override {
+    const ccpi_geometry g = device.geometry(origin, voxel_size, nx, ny, nz);
+    norms_.assign(iterations_, 0.0);
+    check(ccpi_cgls_reg(&g, pixels, voxels, iterations_, kind_, param_, norms_.data(),
+                        flags_, stream));
+    return true;
+  }
+  const std::vector<double> &convergence_data() const { return norms_; }
+
+ private:
+  double param_;
+  int kind_;
+  std::vector<double> norms_;
+};
+
+class cgls_tikhonov : public cgls_regularize {
+ public:
+  cgls_tikhonov(int iterations, double param, int flags = 0)
+      : cgls_regularize(iterations, param, CCPI_REG_TIKHONOV, flags) {}
+};
+
+class cgls_tv_reg : public cgls_regularize {
+ public:
+  cgls_tv_reg(int iterations, double param, int flags = 0)
+      : cgls_regularize(iterations, param, CCPI_REG_TV, flags) {}
+};
+
 class sirt : public reconstruction_alg {
  public:
   using reconstruction_alg::reconstruction_alg;

@@ -75,6 +75,13 @@ def lib():
                 f = getattr(_lib, f"{name}_{sfx}")
                 f.argtypes = [C.POINTER(_Geom), tp, tp, C.c_int]
                 f.restype = C.c_int
+            for name in ("or_tikhonov", "or_tv"):
+                f = getattr(_lib, f"{name}_{sfx}")
+                f.argtypes = [tp, tp, C.c_int, C.c_int, C.c_int]
+                f.restype = None
+            f = getattr(_lib, f"or_cgls_reg_{sfx}")
+            f.argtypes = [C.POINTER(_Geom), tp, tp, C.c_int, C.c_int, C.c_double, dp]
+            f.restype = C.c_int
     return _lib
 
 
@@ -241,3 +248,31 @@ def mlem(geom: Geometry, sino, niter, dtype=np.float32):
     g = geom._struct()
     getattr(lib(), f"or_mlem_{sfx}")(C.byref(g), _ptr(b, ct), _ptr(x, ct), niter)
     return x
+
+
+TIKHONOV, TV = 0, 1
+
+
+def regularize(kind: int, vol, dtype=np.float32):
+    """Tikhonov (kind 0, Core/src/tikhonov.cpp:4-32) or TV (kind 1,
+    Core/src/total_v.cpp:21-195) term of a [nx][ny][nz] volume."""
+    sfx, ct = _sfx(dtype)
+    a = np.ascontiguousarray(vol, dtype=dtype)
+    out = np.empty_like(a)
+    nx, ny, nz = a.shape
+    name = "or_tikhonov" if kind == TIKHONOV else "or_tv"
+    getattr(lib(), f"{name}_{sfx}")(_ptr(a, ct), _ptr(out, ct), nx, ny, nz)
+    return out
+
+
+def cgls_reg(geom: Geometry, sino, niter, kind: int, lam: float, x=None, dtype=np.float32):
+    """Regularised CGLS body (Core/src/Algorithms/cgls.cpp:343-435); returns
+    (x, residual, norms)."""
+    sfx, ct = _sfx(dtype)
+    b = np.array(sino, dtype=dtype, order="C", copy=True)
+    x = np.zeros(geom.shape, dtype) if x is None else np.array(x, dtype=dtype, order="C")
+    norms = np.zeros(max(niter, 1), np.float64)
+    g = geom._struct()
+    getattr(lib(), f"or_cgls_reg_{sfx}")(C.byref(g), _ptr(b, ct), _ptr(x, ct), niter, kind,
+                                         float(lam), _dp(norms))
+    return x, b, norms[:niter]

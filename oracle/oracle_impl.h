@@ -13,6 +13,9 @@
  *   CGLS (cgls_3d)               Core/src/Algorithms/cgls.cpp:26-157
  *   SIRT                         Core/src/Algorithms/sirt.cpp:15-80
  *   MLEM                         Core/src/Algorithms/mlem.cpp:15-80
+ *   Tikhonov / TV terms          Core/src/tikhonov.cpp:4-32,
+ *                                Core/src/total_v.cpp:21-195
+ *   regularised CGLS             Core/src/Algorithms/cgls.cpp:343-435
  * The float instantiation (RT = float) keeps the reference's precision
  * choices: geometry in double, weights stored as float, float accumulators,
  * float CGLS alpha.  The double instantiation (RT = double) is the FP64
@@ -1124,3 +1127,162 @@ int FN(or_mlem)(const or_geom *g, const RT *b, RT *x, int niter)
   free(t);
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* regularisation terms and the regularised CGLS                       */
+/* ------------------------------------------------------------------ */
+
+#define OR_MIRROR(i, n, d) ((i) + (d) >= 0 && (i) + (d) < (n) ? (i) + (d) : (i) - (d))
+
+/* Tikhonov term (tikhonov.cpp:4-32): b = 0.25 (a(i+1) + a(i-1) + a(j+1) +
+   a(j-1)) - a, a neighbour past a face mirrored to the other side; z is not
+   coupled.  The neighbour sum is in RT, the scaling in double (0.25 is a
+   double literal).  Needs nx, ny >= 2. */
+void FN(or_tikhonov)(const RT *a, RT *b, int nx, int ny, int nz)
+{
+  long i;
+#pragma omp parallel for schedule(static)
+  for (i = 0; i < nx; i++) {
+    const long i1 = OR_MIRROR(i, nx, 1), i2 = OR_MIRROR(i, nx, -1);
+    long j, k;
+    for (j = 0; j < ny; j++) {
+      const long j1 = OR_MIRROR(j, ny, 1), j2 = OR_MIRROR(j, ny, -1);
+      const RT *p1 = a + (i1 * ny + j) * nz, *p2 = a + (i2 * ny + j) * nz;
+      const RT *p3 = a + (i * ny + j1) * nz, *p4 = a + (i * ny + j2) * nz;
+      const RT *pc = a + (i * ny + j) * nz;
+      RT *q = b + (i * ny + j) * nz;
+      for (k = 0; k < nz; k++)
+        q[k] = (RT)(0.25 * (double)(p1[k] + p2[k] + p3[k] + p4[k]) - (double)pc[k]);
+    }
+  }
+}
+
+static int FN(or_sgn)(RT x) { return (x > 0) - (x < 0); }
+
+/* 0.5 (sign(p) + sign(m)) min(|p|, |m|) in double, stored in RT
+   (total_v.cpp:62-67). */
+static RT FN(or_minmod)(RT p, RT m)
+{
+  const RT ap = p < 0 ? -p : p, am = m < 0 ? -m : m;
+  return (RT)(0.5 * (double)(FN(or_sgn)(p) + FN(or_sgn)(m)) * (double)(ap < am ? ap : am));
+}
+
+/* TV term (total_v.cpp:21-195): the divergence of the normalised forward
+   differences, D1 = dx+/T1, D2 = dy+/T2, D3 = dz+/T3, with the reference's
+   pairing of min-mod terms in T1..T3 (T2 reuses the z term of T1, T3 the y
+   term of T1), mirrored neighbours at the faces, and
+   b = (D1 - D1(i-1)) + (D2 - D2(j-1)) + (D3 - D3(k-1)).  Needs nx, ny,
+   nz >= 2. */
+void FN(or_tv)(const RT *a, RT *b, int nx, int ny, int nz)
+{
+  const long n = (long)nx * ny * nz;
+  RT *D = (RT *)malloc(sizeof(RT) * 3 * (size_t)n);
+  long i;
+#pragma omp parallel for schedule(static)
+  for (i = 0; i < nx; i++) {
+    const long i1 = OR_MIRROR(i, nx, 1), i2 = OR_MIRROR(i, nx, -1);
+    long j, k;
+    for (j = 0; j < ny; j++) {
+      const long j1 = OR_MIRROR(j, ny, 1), j2 = OR_MIRROR(j, ny, -1);
+      for (k = 0; k < nz; k++) {
+        const long k1 = OR_MIRROR(k, nz, 1), k2 = OR_MIRROR(k, nz, -1);
+        const long c = (i * ny + j) * nz + k;
+        const RT A = a[c];
+        const RT x1 = a[(i1 * ny + j) * nz + k] - A, x0 = A - a[(i2 * ny + j) * nz + k];
+        const RT y1 = a[(i * ny + j1) * nz + k] - A, y0 = A - a[(i * ny + j2) * nz + k];
+        const RT z1 = a[(i * ny + j) * nz + k1] - A, z0 = A - a[(i * ny + j) * nz + k2];
+        const RT d1x = x1 * x1, d1y = y1 * y1, d1z = z1 * z1;
+        RT d2x = FN(or_minmod)(y1, y0), d2y = FN(or_minmod)(x1, x0);
+        RT d3x = FN(or_minmod)(z1, z0);
+        d2x = d2x * d2x;
+        d2y = d2y * d2y;
+        d3x = d3x * d3x;
+        {
+          const RT d2z = d2y, d3y = d3x, d3z = d2x;
+          const RT T1 = (RT)sqrt(d1x + d2x + d3x + (RT)FLT_EPSILON);
+          const RT T2 = (RT)sqrt(d1y + d2y + d3y + (RT)FLT_EPSILON);
+          const RT T3 = (RT)sqrt(d1z + d2z + d3z + (RT)FLT_EPSILON);
+          D[c] = x1 / T1;
+          D[n + c] = y1 / T2;
+          D[2 * n + c] = z1 / T3;
+        }
+      }
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (i = 0; i < nx; i++) {
+    const long i2 = OR_MIRROR(i, nx, -1);
+    long j, k;
+    for (j = 0; j < ny; j++) {
+      const long j2 = OR_MIRROR(j, ny, -1);
+      for (k = 0; k < nz; k++) {
+        const long k2 = OR_MIRROR(k, nz, -1);
+        const long c = (i * ny + j) * nz + k;
+        const RT dv1 = D[c] - D[(i2 * ny + j) * nz + k];
+        const RT dv2 = D[n + c] - D[n + (i * ny + j2) * nz + k];
+        const RT dv3 = D[2 * n + c] - D[2 * n + (i * ny + j) * nz + k2];
+        b[c] = dv1 + dv2 + dv3;
+      }
+    }
+  }
+  free(D);
+}
+
+/* cgls_regularize::reconstruct (cgls.cpp:343-423) with regularize = the
+   Tikhonov (kind 0) or TV (kind 1) term (cgls.cpp:425-435).  Each
+   iteration: Ad = A d; Ad_vox += lambda L(d) (Ad_vox is never reset);
+   alpha = gamma / (||Ad||^2 + ||Ad_vox||^2); x += alpha d; r -= alpha Ad;
+   x += lambda L(x); s = A^T r; beta, d = s + beta d.  The reference also
+   updates an unused voxel residual with the sinogram's extents
+   (cgls.cpp:390), which is undefined behaviour and has no effect on x or
+   the norms; it is left out. */
+int FN(or_cgls_reg)(const or_geom *g, RT *b, RT *x, int niter, int kind,
+                    double lambda, double *norms)
+{
+  const long nvox = (long)g->nx * g->ny * g->nz;
+  const long npix = (long)g->na * g->nh * g->nv;
+  RT *d = (RT *)calloc((size_t)nvox, sizeof(RT));
+  RT *adv = (RT *)calloc((size_t)nvox, sizeof(RT));
+  RT *L = (RT *)malloc(sizeof(RT) * (size_t)nvox);
+  RT *s = (RT *)malloc(sizeof(RT) * (size_t)nvox);
+  RT *Ad = (RT *)malloc(sizeof(RT) * (size_t)npix);
+  const RT lam = (RT)lambda;
+  RT normr2;
+  int it;
+  long q;
+  FN(bp)(g, b, d);
+  normr2 = FN(sumsq)(d, g->nx, g->ny, g->nz);
+  for (it = 0; it < niter; it++) {
+    RT alpha;
+    double nn, beta;
+    memset(Ad, 0, sizeof(RT) * (size_t)npix);
+    FN(fp)(g, d, Ad);
+    if (kind == 0) FN(or_tikhonov)(d, L, g->nx, g->ny, g->nz);
+    else FN(or_tv)(d, L, g->nx, g->ny, g->nz);
+    for (q = 0; q < nvox; q++) adv[q] += lam * L[q];
+    alpha = FN(sumsq)(Ad, g->na, g->nh, g->nv) + FN(sumsq)(adv, g->nx, g->ny, g->nz);
+    alpha = normr2 / alpha;
+    for (q = 0; q < nvox; q++) x[q] += alpha * d[q];
+    for (q = 0; q < npix; q++) b[q] += (-alpha) * Ad[q];
+    if (kind == 0) FN(or_tikhonov)(x, L, g->nx, g->ny, g->nz);
+    else FN(or_tv)(x, L, g->nx, g->ny, g->nz);
+    for (q = 0; q < nvox; q++) x[q] += lam * L[q];
+    memset(s, 0, sizeof(RT) * (size_t)nvox);
+    FN(bp)(g, b, s);
+    nn = (double)FN(sumsq)(s, g->nx, g->ny, g->nz);
+    beta = nn / (double)normr2;
+    normr2 = (RT)nn;
+    {
+      const RT bf = (RT)beta;
+      for (q = 0; q < nvox; q++) d[q] = bf * d[q] + s[q];
+    }
+    if (norms) norms[it] = sqrt((double)normr2);
+  }
+  free(d);
+  free(adv);
+  free(L);
+  free(s);
+  free(Ad);
+  return 0;
+}
+#undef OR_MIRROR

@@ -1,6 +1,7 @@
 // C++ caller of the drop-in facade (include/ccpi_hip.hpp): a small cone-beam
-// FP, BP and CGLS through the reference-shaped classes.  Prints one line
-// "facade ok <fp_sum> <bp_sum> <norm0>" or exits non-zero.
+// FP, BP, CGLS and Tikhonov-regularised CGLS through the reference-shaped
+// classes.  Prints one line "facade ok <fp_sum> <bp_sum> <norm0> <tik_norm0>"
+// or exits non-zero.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -42,7 +43,12 @@ int main() {
     hipMemset(dx, 0, nvox * 4);
     ccpi::hip::cgls_3d alg(3);
     alg.reconstruct(dev, dy, dx, origin, width, n, n, n);
-    std::printf("facade ok %.6g %.6g %.6g\n", fps, bps, alg.convergence_data()[0]);
+    hipMemcpy(dy, y.data(), npix * 4, hipMemcpyHostToDevice);
+    hipMemset(dx, 0, nvox * 4);
+    ccpi::hip::cgls_tikhonov tik(3, 0.2);
+    tik.reconstruct(dev, dy, dx, origin, width, n, n, n);
+    std::printf("facade ok %.9g %.9g %.9g %.9g\n", fps, bps, alg.convergence_data()[0],
+                tik.convergence_data()[0]);
   } catch (const ccpi::hip::error &e) {
     std::printf("facade error %d %s\n", e.status, e.what());
     return 1;

@@ -1,5 +1,6 @@
 """C++ callers: the reference-shaped facade (include/ccpi_hip.hpp) drives
-FP, BP and CGLS through the C ABI (GPU test; binary built by build())."""
+FP, BP, CGLS and cgls_tikhonov through the C ABI (GPU test; binary built by
+build())."""
 
 import os
 import subprocess
@@ -20,7 +21,7 @@ def test_cxx_facade_matches_oracle():
     assert out.returncode == 0, out.stdout + out.stderr
     parts = out.stdout.split()
     assert parts[:2] == ["facade", "ok"]
-    fps, bps, norm0 = (float(q) for q in parts[2:5])
+    fps, bps, norm0, tik0 = (float(q) for q in parts[2:6])
     n, na, nh, nv = 10, 12, 16, 14
     og = oracle.Geometry(2 * np.pi * np.arange(na) / na, (np.arange(nh) - 7.5) * 1.5,
                          (np.arange(nv) - 6.5) * 1.5, (-5, -5, -5), (1, 1, 1), (n, n, n),
@@ -31,3 +32,6 @@ def test_cxx_facade_matches_oracle():
     assert abs(bps - x.sum()) < 1e-5 * x.sum()
     _, _, norms = oracle.cgls(og, y.astype(np.float32), 3, dtype=np.float64)
     assert abs(norm0 - norms[0]) < 1e-4 * norms[0]
+    _, _, tn = oracle.cgls_reg(og, y.astype(np.float32), 3, oracle.TIKHONOV, 0.2,
+                               dtype=np.float64)
+    assert abs(tik0 - tn[0]) < 1e-4 * tn[0]

@@ -78,3 +78,23 @@ def test_cone_geometry_rejected():
     # source inside the volume's rotation circle
     assert L.ccpi_cb_f2D(-2.0, 0.0, 0.0, 50.0, _dp(h), _dp(v), _dp(ang), dummy, dummy,
                          1, 2, 2, _dp(o), _dp(s), 4, 4, 4, None) == -2
+
+
+def test_regularize_rejects_bad_arguments():
+    """Mirrored neighbours need every extent >= 2; unknown kinds and
+    in-place calls are refused before any device work."""
+    L = _lib.lib()
+    a, b = C.c_void_p(16), C.c_void_p(32)
+    assert L.ccpi_regularize(_lib.CCPI_REG_TV, a, b, 4, 4, 1, None) == -1
+    assert L.ccpi_regularize(_lib.CCPI_REG_TIKHONOV, a, b, 1, 4, 4, None) == -1
+    assert L.ccpi_regularize(7, a, b, 4, 4, 4, None) == -1
+    assert L.ccpi_regularize(_lib.CCPI_REG_TV, a, a, 4, 4, 4, None) == -1
+    assert L.ccpi_regularize(_lib.CCPI_REG_TV, None, b, 4, 4, 4, None) == -1
+    assert b"extent" in L.ccpi_last_error() or b"null" in L.ccpi_last_error()
+
+
+def test_cgls_reg_rejects_unknown_kind():
+    L = _lib.lib()
+    dummy = C.c_void_p(16)
+    assert L.ccpi_cgls_reg(None, dummy, dummy, 1, 5, 0.1, None, 0, None) == -1
+    assert L.ccpi_cgls_reg(None, dummy, dummy, -1, 0, 0.1, None, 0, None) == -1
