@@ -148,6 +148,82 @@ def cgls_TVreg(pixels, angles, h_offsets, v_offsets, pixels_per_voxel, source_to
     return out
 
 
+# XTek phantom scan (Readers/xtek.cpp:131-173, 392-457)
+PHANTOM_N = 512
+PHANTOM_PIXEL = 0.390625
+PHANTOM_CUBES = (((107, 189), (107, 189), (57, 139)),
+                 ((189, 271), (189, 271), (139, 221)),
+                 ((271, 353), (271, 353), (221, 303)))
+
+
+def phantom_setup():
+    """The phantom scan's geometry (xtek.cpp:131-173): source at x = -250,
+    detector at x = 737, 512 x 512 pixels of 0.390625 mm centred on the
+    axis, 250 angles over [0, 2 pi), mask radius from the outermost pixel.
+    Returns (geometry of the 512^3 phantom volume, angles in radians,
+    mask radius); the volume's voxel size is 2 mask_radius / 512 and it is
+    centred on the origin (xtek.cpp:392-414)."""
+    src_x, det_x = -250.0, 737.0
+    first = -99.8046875
+    c = 0
+    while c < 1000 and first + c * PHANTOM_PIXEL < 100.0 + 0.001:
+        c += 1
+    pix = first + np.arange(c, dtype=np.float64) * PHANTOM_PIXEL
+    na = 250
+    phi = np.arange(na, dtype=np.float64) * (2.0 * np.pi / na)
+    mask = -src_x * np.sin(np.arctan(pix[-1] / (det_x - src_x)))
+    n = PHANTOM_N
+    vox = 2.0 * mask / n
+    off = -(vox * n) / 2.0
+    g = _pj.Geometry(phi, pix, pix.copy(), (off, off, off), (vox, vox, vox), (n, n, n),
+                     cone=True, source=(src_x, 0.0, 0.0), detector_x=det_x)
+    return g, phi, float(mask)
+
+
+def _phantom_volume(dev):
+    """512^3 zeros with three unit cubes (xtek.cpp:424-446)."""
+    import torch
+    n = PHANTOM_N
+    x = torch.zeros((n, n, n), dtype=torch.float32, device=dev)
+    for (i0, i1), (j0, j1), (k0, k1) in PHANTOM_CUBES:
+        x[i0:i1, j0:j1, k0:k1] = 1.0
+    return x
+
+
+def _phantom_scalars(g, mask):
+    # the wrapper reports source_to_sample / source_to_detector
+    # (conebeam_wrapper.cpp:245-248)
+    sx = -g.source[0]
+    return sx, g.detector_x + sx, PHANTOM_PIXEL, PHANTOM_PIXEL, mask
+
+
+def create_phantom():
+    """Projections of the XTek cube phantom (conebeam_wrapper.cpp:238-268):
+    (pixels [250][512][512] as [angle][h][v], angles in degrees (float32),
+    source_to_sample 250.0, source_to_detector 987.0, pixel_h_size,
+    pixel_v_size, mask_radius).  The projections come from this library's
+    cone forward projector; the reference uses its generic 3-D Siddon
+    (project_line.hpp) for this one call.  Both give exact ray-voxel
+    intersection lengths."""
+    g, phi, mask = phantom_setup()
+    dev = _device()
+    y = _pj.forward_project(g, _phantom_volume(dev))
+    ang = (phi * 180.0 / np.pi).astype(np.float32)
+    return (y.cpu().numpy(), ang) + _phantom_scalars(g, mask)
+
+
+def create_phantom_volume():
+    """The phantom volume itself (conebeam_wrapper.cpp:270-311): (voxels
+    [512][512][512], angles in degrees, h_pixels, v_pixels (float32),
+    source_to_sample, source_to_detector, pixel_h_size, pixel_v_size,
+    mask_radius)."""
+    g, phi, mask = phantom_setup()
+    vol = _phantom_volume(_device()).cpu().numpy()
+    ang = (phi * 180.0 / np.pi).astype(np.float32)
+    return ((vol, ang, g.h_pixels.astype(np.float32), g.v_pixels.astype(np.float32))
+            + _phantom_scalars(g, mask))
+
+
 def last_geometry():
     """Geometry (with the find_centre shift) of the last reconstruction."""
     return _last.get("geometry")
@@ -174,5 +250,5 @@ def create_sinogram(source_x, source_y, source_z, detector_x, h_pixels, v_pixels
 
 
 for _f in (cgls, cgls_conv, sirt, mlem, cgls_tikhonov, cgls_TVreg, last_geometry, cgls_norms,
-           create_sinogram):
+           create_sinogram, create_phantom, create_phantom_volume):
     setattr(alg, _f.__name__, _f)

@@ -111,3 +111,27 @@ def cgls_textbook(A, b, niter):
         xs.append(x.copy())
         norms.append(np.sqrt(gamma))
     return xs, np.array(norms)
+
+
+def cone_box_lengths(source, det_x, angle, h_pix, v_pix, boxes):
+    """Exact length (mm) inside a set of disjoint axis-aligned boxes
+    ((lo_x, lo_y, lo_z), (hi_x, hi_y, hi_z)) of every source -> pixel
+    segment of one angle, as [nh][nv] in float64 (same ray convention as
+    cone_matrix)."""
+    c, s = np.cos(angle), np.sin(angle)
+    sx, sy, sz = source
+    p1 = (c * sx - s * sy, s * sx + c * sy, sz)
+    H, V = np.meshgrid(np.asarray(h_pix, np.float64), np.asarray(v_pix, np.float64),
+                       indexing="ij")
+    p2 = (c * det_x - s * H, s * det_x + c * H, V)
+    d = [p2[q] - p1[q] for q in range(3)]
+    for q in range(3):
+        d[q] = np.where(np.abs(d[q]) < 1e-12, 0.0, d[q])
+    norm = np.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2)
+    total = np.zeros(H.shape)
+    for lo, hi in boxes:
+        t = [_slab(p1[q], d[q], lo[q], hi[q]) for q in range(3)]
+        t0 = np.maximum(np.maximum(np.maximum(t[0][0], t[1][0]), t[2][0]), 0.0)
+        t1 = np.minimum(np.minimum(np.minimum(t[0][1], t[1][1]), t[2][1]), 1.0)
+        total += np.maximum(0.0, t1 - t0) * norm
+    return total
