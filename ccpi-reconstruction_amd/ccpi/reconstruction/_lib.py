@@ -25,6 +25,7 @@ EXPORTS = (
     "ccpi_pb_f2D", "ccpi_pb_b2D", "ccpi_cb_f2D", "ccpi_cb_b2D",
     "ccpi_forward_project", "ccpi_backward_project",
     "ccpi_cgls", "ccpi_sirt", "ccpi_mlem", "ccpi_regularize", "ccpi_cgls_reg",
+    "ccpi_xtek_find_centre",
     "ccpi_pb_cgls_batched", "ccpi_cgls_create", "ccpi_cgls_iterate", "ccpi_cgls_norms", "ccpi_cgls_timings",
     "ccpi_cgls_destroy", "ccpi_projector_nnz",
     "ccpi_error_string", "ccpi_last_error", "ccpi_version",
@@ -81,6 +82,7 @@ def lib():
     L.ccpi_mlem.argtypes = [gp, fp, fp, i, i, vp]
     L.ccpi_regularize.argtypes = [i, fp, fp, i, i, i, vp]
     L.ccpi_cgls_reg.argtypes = [gp, fp, fp, i, i, d, dp, i, vp]
+    L.ccpi_xtek_find_centre.argtypes = [gp, fp, i, dp, vp]
     L.ccpi_pb_cgls_batched.argtypes = [gp, i, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                        i, dp, i, vp]
     ip = C.POINTER(C.c_int)
@@ -93,7 +95,7 @@ def lib():
     for name in ("ccpi_pb_f2D", "ccpi_pb_b2D", "ccpi_cb_f2D", "ccpi_cb_b2D",
                  "ccpi_forward_project", "ccpi_backward_project", "ccpi_cgls",
                  "ccpi_sirt", "ccpi_mlem", "ccpi_regularize", "ccpi_cgls_reg",
-                 "ccpi_pb_cgls_batched", "ccpi_cgls_create",
+                 "ccpi_xtek_find_centre", "ccpi_pb_cgls_batched", "ccpi_cgls_create",
                  "ccpi_cgls_iterate", "ccpi_cgls_norms", "ccpi_cgls_timings",
                  "ccpi_cgls_destroy", "ccpi_projector_nnz"):
         getattr(L, name).restype = C.c_int

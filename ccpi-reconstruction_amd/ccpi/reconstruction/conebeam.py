@@ -52,7 +52,7 @@ def _reconstruct(algo, pixels, angles, h_offsets, v_offsets, pixels_per_voxel,
                             float(pixel_h_size), float(pixel_v_size), float(mask_radius))
     b = _ins.xtek_read_scans(torch.from_numpy(pix).to(dev), setup, bool(is_pixels_in_log))
     g = setup.geometry
-    _ins.find_centre(setup, b[:, :, g.v_pixels.size // 2].cpu().numpy())
+    _ins.find_centre(setup, b, g.v_pixels.size // 2)
     x = torch.zeros(g.shape, dtype=torch.float32, device=dev)
     if algo == "cgls":
         x, _, norms = _pj.cgls(g, b, int(niterations), x)

@@ -11,7 +11,7 @@ Restates what the reference does between the Python call and the solver:
   v padding                calc_v_alignment, Core/src/instruments.cpp:35-63
   block sizes              calculate_block_sizes, Core/src/voxels.cpp:12-46
 Geometry is host-side (double); the scan transpose / -log / padding run on
-the device with torch.
+the device with torch, and find_centre's search runs in the HIP library.
 """
 
 from __future__ import annotations
@@ -166,78 +166,20 @@ def xtek_read_scans(pixels, setup: Setup, is_log: bool):
     return out.contiguous()
 
 
-def _interp2d(h, ang, data, new_h, new_a):
-    """Vectorised interpolate2D (xtek.cpp:566-669): data [na][nh] at
-    positions (new_h, new_a); angles wrap at 2 pi."""
-    nh, na = h.size, ang.size
-    ph1 = np.minimum(np.searchsorted(h[1:], new_h, side="right"), nh - 1)
-    exact_h = h[ph1] == new_h
-    ph2 = np.minimum(ph1 + 1, nh - 1)
-    two_pi = 2.0 * math.pi
-    below = new_a < ang[0]
-    pa1 = np.minimum(np.searchsorted(ang[1:], new_a, side="right"), na - 1)
-    exact_a = (~below) & (ang[pa1] == new_a)
-    last = (~below) & (pa1 == na - 1)
-    a_lo = np.where(below, na - 1, pa1)
-    a_hi = np.where(below | last, 0, np.minimum(pa1 + 1, na - 1))
-    x1 = np.where(below, ang[na - 1] - two_pi, ang[a_lo])
-    x2 = np.where(below, ang[0], np.where(last, ang[0] + two_pi, ang[a_hi]))
-    f_lo_1 = data[a_lo, ph1].astype(np.float64)
-    f_hi_1 = data[a_hi, ph1].astype(np.float64)
-    f_lo_2 = data[a_lo, ph2].astype(np.float64)
-    f_hi_2 = data[a_hi, ph2].astype(np.float64)
-    hx1, hx2 = h[ph1], h[ph2]
-    with np.errstate(divide="ignore", invalid="ignore"):
-        lin_a = (f_lo_1 * (x2 - new_a) + f_hi_1 * (new_a - x1)) / (x2 - x1)
-        lin_h = (f_lo_1 * (hx2 - new_h) + f_lo_2 * (new_h - hx1)) / (hx2 - hx1)
-        bil = (f_lo_1 * (hx2 - new_h) * (x2 - new_a) + f_lo_2 * (new_h - hx1) * (x2 - new_a)
-               + f_hi_1 * (hx2 - new_h) * (new_a - x1)
-               + f_hi_2 * (new_h - hx1) * (new_a - x1)) / ((hx2 - hx1) * (x2 - x1))
-    out = np.where(exact_h, np.where(exact_a, f_lo_1, lin_a),
-                   np.where(exact_a, lin_h, bil))
-    return out.astype(np.float32)
-
-
-def find_centre(setup: Setup, sino_slice: np.ndarray) -> float:
+def find_centre(setup: Setup, sino, v_slice: int) -> float:
     """Centre-of-rotation search of xtek.cpp:671-758 (T. Liu, Opt. Eng. 2009,
-    046501) on one detector row ``sino_slice`` [angle][h]; shifts the
-    geometry's source y and h pixels in place and returns the shift (mm)."""
+    046501) on row ``v_slice`` of the device sinogram ``sino``
+    [angle][h][v], run by the HIP library (ccpi_xtek_find_centre).  Shifts
+    the geometry's source y and h pixels in place (xtek.cpp:755-756) and
+    returns the shift (mm)."""
+    import ctypes as C
+    from ._lib import check, lib
+    from .projectors import _dev, _stream
     g = setup.geometry
-    h, ph = g.h_pixels, g.angles
-    na = ph.size
-    sx = g.source[0]
-    distance = g.detector_x - sx
-    data = np.asarray(sino_slice, np.float32)
-    midpoint = 0.0
-    nsamples = 21
-    gamma_i = np.arctan(h / distance)
-    for prec in (1.0, 0.1, 0.01, 0.001, 0.0001):
-        scor = midpoint - (nsamples // 2) * prec
-        M = np.zeros(nsamples, np.float32)
-        for j in range(nsamples):
-            gamma_c = math.atan((midpoint + (j - nsamples // 2) * prec) / distance)
-            beta = math.pi + 2.0 * (gamma_i - gamma_c)
-            s2 = distance * np.tan(2.0 * gamma_c - gamma_i)
-            ks = np.nonzero((s2 >= h[0]) & (s2 <= h[-1]))[0]
-            if ks.size == 0:
-                M[j] = np.float32(np.nan)
-                continue
-            ab = ph[:, None] + beta[None, ks]
-            # wrap into [ph[0], ph[na-1]] like the reference's two loops
-            two_pi = 2.0 * math.pi
-            lo = ab < ph[0]
-            ab = np.where(lo, ab + two_pi * np.ceil((ph[0] - ab) / two_pi), ab)
-            hi = ab > ph[na - 1]
-            ab = np.where(hi, ab - two_pi * np.ceil((ab - ph[na - 1]) / two_pi), ab)
-            p = _interp2d(h, ph, data, np.broadcast_to(s2[ks], ab.shape), ab)
-            t = data[:, ks] - p
-            M[j] = np.float32(np.sum(t * t, dtype=np.float32) / np.float32(t.size))
-        valid = np.where(np.isnan(M), np.inf, M)
-        ind = int(np.argmin(valid))
-        if not np.isfinite(valid[ind]):
-            _fail("Failure in XTek find centre")
-        midpoint = scor + ind * prec
-    y_centre = midpoint * sx / distance
+    y = C.c_double()
+    check(lib().ccpi_xtek_find_centre(C.byref(g.c_struct()), _dev(sino, g.sino_shape, "sino"),
+                                      int(v_slice), C.byref(y), _stream()))
+    y_centre = y.value
     g.source = (g.source[0], g.source[1] + y_centre, g.source[2])
     g.h_pixels = g.h_pixels + y_centre
     g._c = None

@@ -155,6 +155,15 @@ int ccpi_cgls_reg(const ccpi_geometry *g, float *b, float *x, int iterations, in
   });
 }
 
+int ccpi_xtek_find_centre(const ccpi_geometry *g, const float *pixels, int v_slice,
+                          double *y_centre, void *stream) {
+  return guarded([&] {
+    CCPI_REQUIRE(pixels && y_centre, CCPI_E_ARG, "null data pointer");
+    const ccpi::Geometry G = ccpi::Geometry::from(g);
+    *y_centre = ccpi::xtek_find_centre(G, pixels, v_slice, S(stream));
+  });
+}
+
 int ccpi_sirt(const ccpi_geometry *g, const float *b, float *x, int iterations, int flags,
               void *stream) {
   (void)flags;

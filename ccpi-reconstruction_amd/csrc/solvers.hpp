@@ -64,6 +64,9 @@ struct CglsRun {
 void dev_fill(float *p, size_t n, float v, hipStream_t st);
 void run_cgls(const Projector &P, float *b, float *x, int iters, double *norms,
               bool fp32, Workspace &W, hipStream_t st);
+// centre.hip: XTek centre-of-rotation search (xtek.cpp:671-758) on row
+// v_slice of the device sinogram; returns the y shift of source and pixels
+double xtek_find_centre(const Geometry &G, const float *pixels, int v_slice, hipStream_t st);
 // regularised CGLS (cgls.cpp:343-435), kind kRegTikhonov or kRegTV
 void run_cgls_reg(const Projector &P, float *b, float *x, int iters, int kind, double lambda,
                   double *norms, bool fp32, Workspace &W, hipStream_t st);

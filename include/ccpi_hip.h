@@ -134,6 +134,17 @@ int ccpi_cgls_reg(const ccpi_geometry *g, float *b, float *x, int iterations,
                   int kind, double lambda, double *norms, int flags,
                   void *stream);
 
+/* ---- ingest -------------------------------------------------------------
+   Nikon_XTek::find_centre(v_slice) (Core/src/Readers/xtek.cpp:671-758):
+   centre-of-rotation search on row v_slice of the device sinogram
+   pixels [n_angles][n_h][n_v] of a cone geometry.  *y_centre receives the
+   shift the reference then adds to source_y and to every h pixel
+   (xtek.cpp:755-756); the geometry itself is not modified.  Fails with
+   CCPI_E_GEOMETRY where the reference reports "Failure in XTek find
+   centre". */
+int ccpi_xtek_find_centre(const ccpi_geometry *g, const float *pixels,
+                          int v_slice, double *y_centre, void *stream);
+
 /* ---- CGLS session: the same body split into launch-only steps ----------
    No counterpart in the reference (its loop is internal to
    cgls_base::reconstruct, cgls.cpp:54-82).  For callers that interleave

@@ -80,6 +80,9 @@ def lib():
                 f.argtypes = [tp, tp, C.c_int, C.c_int, C.c_int]
                 f.restype = None
             f = getattr(_lib, f"or_cgls_reg_{sfx}")
+            _lib.or_find_centre.argtypes = [C.POINTER(_Geom), C.POINTER(C.c_float), C.c_int,
+                                            C.POINTER(C.c_double), C.POINTER(C.c_double)]
+            _lib.or_find_centre.restype = C.c_int
             f.argtypes = [C.POINTER(_Geom), tp, tp, C.c_int, C.c_int, C.c_double, dp]
             f.restype = C.c_int
     return _lib
@@ -276,3 +279,18 @@ def cgls_reg(geom: Geometry, sino, niter, kind: int, lam: float, x=None, dtype=n
     getattr(lib(), f"or_cgls_reg_{sfx}")(C.byref(g), _ptr(b, ct), _ptr(x, ct), niter, kind,
                                          float(lam), _dp(norms))
     return x, b, norms[:niter]
+
+
+def find_centre(geom: Geometry, pixels, v_slice: int):
+    """XTek centre-of-rotation search (Core/src/Readers/xtek.cpp:671-758) on
+    row v_slice of a [angle][h][v] float sinogram.  Returns (y_centre,
+    M[5][21]), or (None, M) when the search fails."""
+    p = np.ascontiguousarray(pixels, dtype=np.float32)
+    if p.shape != geom.sino_shape:
+        raise ValueError(f"pixels must be {geom.sino_shape}")
+    g = geom._struct()
+    y = C.c_double()
+    m = np.zeros((5, 21), np.float64)
+    st = lib().or_find_centre(C.byref(g), p.ctypes.data_as(C.POINTER(C.c_float)), int(v_slice),
+                              C.byref(y), _dp(m))
+    return (None if st else y.value), m
