@@ -224,6 +224,15 @@ def create_phantom_volume():
             + _phantom_scalars(g, mask))
 
 
+def load_xtek(filename):
+    """Read a Nikon XTek scan from its .xtekct file
+    (conebeam_wrapper.cpp:313-344): (pixels [angle][h][v] after -log,
+    angles in degrees, source_to_sample, source_to_detector, 0.2, 0.2,
+    mask_radius); see ``formats.load_xtek``."""
+    from . import formats
+    return formats.load_xtek(filename)
+
+
 def last_geometry():
     """Geometry (with the find_centre shift) of the last reconstruction."""
     return _last.get("geometry")
@@ -250,5 +259,5 @@ def create_sinogram(source_x, source_y, source_z, detector_x, h_pixels, v_pixels
 
 
 for _f in (cgls, cgls_conv, sirt, mlem, cgls_tikhonov, cgls_TVreg, last_geometry, cgls_norms,
-           create_sinogram, create_phantom, create_phantom_volume):
+           create_sinogram, create_phantom, create_phantom_volume, load_xtek):
     setattr(alg, _f.__name__, _f)
