@@ -137,3 +137,26 @@ def test_backprojection_function_backward_is_forward():
     w = torch.tensor(rng.random(c.shape), dtype=torch.float32, device="cuda")
     (backproject(y, g) * w).sum().backward()
     assert torch.allclose(y.grad, forward_project(g, w), rtol=1e-5, atol=1e-6)
+
+
+def test_iterative_reconstructor_front_end():
+    """Reconstructor.create(IterativeReconstructor, ...) runs the chosen
+    parallelbeam.alg solver with the given parameters (Reconstructor.py:
+    114-160) and, without a centre, estimates one with find_center_vo."""
+    from ccpi.reconstruction.parallelbeam import alg
+    from ccpi.reconstruction.Reconstructor import IterativeReconstructor, Reconstructor
+    sino = np.load(os.path.join(GOLDEN, "sino.npy")).astype(np.float32)
+    angles = np.load(os.path.join(GOLDEN, "angles.npy")).astype(np.float32)
+    r = Reconstructor.create(IterativeReconstructor, algorithm=alg.cgls_tikhonov,
+                             normalized_projections=sino, angles=angles,
+                             center_of_rotation=80.0, iterations=3,
+                             regularization_parameter=0.05)
+    vol = r.iterate()
+    want = alg.cgls_tikhonov(sino, angles, 80.0, 1, 3, 1, 0.05, np.zeros(3), False)
+    assert np.array_equal(vol, want)
+    r = Reconstructor.create(IterativeReconstructor, algorithm=alg.sirt,
+                             normalized_projections=sino, angles=angles, iterations=2)
+    vol = r.iterate()
+    cor = r.getParameter("center_of_rotation")
+    assert 40.0 < cor < 120.0
+    assert vol.shape[2] == 8 and np.isfinite(vol).all()
