@@ -192,7 +192,13 @@ def parallel_cfg1(cpu: bool):
         tc = time.perf_counter() - t0
         out["cpu_oracle_iter_s"] = 20.0 / tc
         out["cpu_cores"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        out["max_abs_diff_vs_cpu_oracle"] = float(np.abs(x.cpu().numpy() - xo).max())
+        xg = x.cpu().numpy()
+        out["max_abs_diff_vs_cpu_oracle"] = float(np.abs(xg - xo).max())
+        # error against the FP64 oracle of the same model (relative l2)
+        x64, _, _ = oracle.cgls(og, bn.astype(np.float64), 20, dtype=np.float64)
+        nrm = float(np.linalg.norm(x64))
+        out["rel_err_vs_fp64_oracle"] = float(np.linalg.norm(xg - x64)) / nrm
+        out["cpu_float_oracle_rel_err_vs_fp64"] = float(np.linalg.norm(xo - x64)) / nrm
     return out
 
 
