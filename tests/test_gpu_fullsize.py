@@ -63,3 +63,41 @@ def test_adjoint_scaling_and_determinism(cone512):
     del y
     assert torch.equal(backward_project(g, 2 * s), 2 * b)
     assert torch.equal(backward_project(g, s), b)
+
+
+def test_parallel_config1_size_parity_and_chords():
+    """The plain parallel-beam workload (256 x 256 x 16, 180 angles over
+    [0, 180 deg), 367 x 16 detector): FP and BP against the oracle (1e-5 of
+    the maximum, the summation-order tolerance of test_gpu_parity) and A 1
+    against the closed-form chord of each ray through the 256 mm square
+    (rays along an edge of the square excepted)."""
+    import oracle
+    from ccpi.reconstruction import workloads as W
+    from ccpi.reconstruction.projectors import backward_project, forward_project
+    g = W.parallel_geometry(256, 16, 180, 367)
+    og = oracle.Geometry(g.angles, g.h_pixels, g.v_pixels, g.origin, g.voxel_size, g.shape)
+    rng = np.random.default_rng(13)
+    x = rng.random(g.shape).astype(np.float32)
+    s = rng.random(g.sino_shape).astype(np.float32)
+    print("\nparallel 256x256x16 parity", flush=True)
+    y = forward_project(g, torch.from_numpy(x).cuda()).cpu().numpy()
+    yo = og.fp(x)
+    assert np.abs(y - yo).max() < 1e-5 * np.abs(yo).max()
+    b = backward_project(g, torch.from_numpy(s).cuda()).cpu().numpy()
+    bo = og.bp(s)
+    assert np.abs(b - bo).max() < 1e-5 * np.abs(bo).max()
+    ones = forward_project(g, torch.ones(g.shape, device="cuda")).double().cpu().numpy()
+    half = 128.0
+    for a in (0, 45, 90, 131):
+        c, sn = np.cos(g.angles[a]), np.sin(g.angles[a])
+        # ray through (-sn h, c h) along (c, sn): slab intervals of the square
+        h = g.h_pixels
+        t = [bf._slab(-sn * h, c, -half, half), bf._slab(c * h, sn, -half, half)]
+        chord = np.maximum(0.0, np.minimum(t[0][1], t[1][1]) - np.maximum(t[0][0], t[1][0]))
+        chord[~np.isfinite(chord)] = 0.0
+        # a ray running along an edge of the square is a tie of the model
+        # (half of its length is credited to the outside row of cells)
+        edge = ((abs(sn) < 1e-9) & (np.abs(np.abs(c * h) - half) < 1e-9)) | \
+               ((abs(c) < 1e-9) & (np.abs(np.abs(sn * h) - half) < 1e-9))
+        dev = np.abs(ones[a] - chord[:, None])[~edge]
+        assert dev.max() < 1e-5 * chord.max()
