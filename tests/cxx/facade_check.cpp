@@ -6,9 +6,19 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "ccpi_hip.hpp"
+
+// HIP runtime calls of this check; a failure exits with status 3
+#define CK(call)                                                        \
+  do {                                                                  \
+    if ((call) != hipSuccess) {                                         \
+      std::printf("facade hip error at line %d\n", __LINE__);           \
+      std::exit(3);                                                     \
+    }                                                                   \
+  } while (0)
 
 int main() {
   const int n = 10, na = 12, nh = 16, nv = 14;
@@ -27,24 +37,24 @@ int main() {
   std::vector<float> x(nvox, 1.0f);
   float *dx = nullptr, *dy = nullptr;
   if (hipMalloc(&dx, nvox * 4) != hipSuccess || hipMalloc(&dy, npix * 4) != hipSuccess) return 2;
-  hipMemcpy(dx, x.data(), nvox * 4, hipMemcpyHostToDevice);
-  hipMemset(dy, 0, npix * 4);
+  CK(hipMemcpy(dx, x.data(), nvox * 4, hipMemcpyHostToDevice));
+  CK(hipMemset(dy, 0, npix * 4));
   try {
     dev.forward_project(dy, dx, origin, width, n, n, n);
     std::vector<float> y(npix);
-    hipMemcpy(y.data(), dy, npix * 4, hipMemcpyDeviceToHost);
+    CK(hipMemcpy(y.data(), dy, npix * 4, hipMemcpyDeviceToHost));
     double fps = 0;
     for (float q : y) fps += q;
-    hipMemset(dx, 0, nvox * 4);
+    CK(hipMemset(dx, 0, nvox * 4));
     dev.backward_project(dy, dx, origin, width, n, n, n);
-    hipMemcpy(x.data(), dx, nvox * 4, hipMemcpyDeviceToHost);
+    CK(hipMemcpy(x.data(), dx, nvox * 4, hipMemcpyDeviceToHost));
     double bps = 0;
     for (float q : x) bps += q;
-    hipMemset(dx, 0, nvox * 4);
+    CK(hipMemset(dx, 0, nvox * 4));
     ccpi::hip::cgls_3d alg(3);
     alg.reconstruct(dev, dy, dx, origin, width, n, n, n);
-    hipMemcpy(dy, y.data(), npix * 4, hipMemcpyHostToDevice);
-    hipMemset(dx, 0, nvox * 4);
+    CK(hipMemcpy(dy, y.data(), npix * 4, hipMemcpyHostToDevice));
+    CK(hipMemset(dx, 0, nvox * 4));
     ccpi::hip::cgls_tikhonov tik(3, 0.2);
     tik.reconstruct(dev, dy, dx, origin, width, n, n, n);
     std::printf("facade ok %.9g %.9g %.9g %.9g\n", fps, bps, alg.convergence_data()[0],
@@ -53,7 +63,7 @@ int main() {
     std::printf("facade error %d %s\n", e.status, e.what());
     return 1;
   }
-  hipFree(dx);
-  hipFree(dy);
+  CK(hipFree(dx));
+  CK(hipFree(dy));
   return 0;
 }
