@@ -248,6 +248,32 @@ __device__ inline int med3_i32(int x, int lo, int hi) {
   return r;
 }
 
+// v_med3_i32 with lo = 0 (inline constant) and a uniform hi: no copy of hi
+// into a VGPR
+__device__ inline int med3_0s(int x, int hi) {
+  int r;
+  asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(x), "s"(hi));
+  return r;
+}
+
+// LDS byte address of a shared-memory pointer, and back
+typedef __attribute__((address_space(3))) float lds_f32;
+__device__ inline uint32_t lds_addr(const float *p) {
+  return (uint32_t)(uintptr_t)(const lds_f32 *)p;
+}
+__device__ inline lds_f32 *lds_ptr(uint32_t a) { return (lds_f32 *)(uintptr_t)a; }
+// (k << 2) + base in one v_lshl_add_u32 (per-lane / uniform base)
+__device__ inline uint32_t k4_add(int k, uint32_t base) {
+  uint32_t r;
+  asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(r) : "v"(k), "v"(base));
+  return r;
+}
+__device__ inline uint32_t k4_add_s(int k, uint32_t base) {
+  uint32_t r;
+  asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(r) : "v"(k), "s"(base));
+  return r;
+}
+
 // min(a, b) for a uniform a as one v_min_f32 (the select form needs a
 // copy of a into a VGPR); operands are finite alphas, so this is the
 // reference's `a < b ? a : b`.
@@ -661,8 +687,9 @@ __device__ inline BpSlot bp_slot(const ConeArgs &g, int v, int pass, bool one_se
 //
 // Accumulation: S bounds the rows per slice, so the 64 rows of one slot
 // fall on distinct slices, and so do their second slices; slot t has its own
-// LDS array, so a slot's first-slice update is a plain read-add-write over
-// distinct cells, followed by its second-slice update.  Cells 0 and nz + 1
+// LDS array, so the slots' first-slice updates are plain read-add-writes
+// over distinct cells, issued together, followed by their second-slice
+// updates.  Cells 0 and nz + 1
 // of an array are trash for slices outside the volume.  ATOMIC (rows per
 // slice beyond the stride bound, or arrays beyond the LDS budget): S = 1,
 // one array, ds_add_f32.
@@ -719,17 +746,19 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
       // k is in [0, nz) (the clamp only guards float ties) and k +- 1 in
       // [-1, nz], where cells -1 and nz are trash.  No row-set tests.
       float dzv[S], izv[S];
-      int d4[S];
-      const float *zb[S];
+      uint32_t b1[S], bz[S];
+      uint32_t b0[S];  // uniform
 #pragma unroll
       for (int t = 0; t < S; ++t) {
         const int v = vb + S * lane + t;
         const bool lower = v < g.mid;
         dzv[t] = g.dz[v];
         izv[t] = g.iz[v];
-        d4[t] = lower ? -4 : 4;
-        zb[t] = s_vz1 + (lower ? 0 : 1);
+        b0[t] = __builtin_amdgcn_readfirstlane(lds_addr(acc + t * astride + 1));
+        b1[t] = b0[t] + (lower ? -4 : 4);
+        bz[t] = lds_addr(s_vz1 + (lower ? 0 : 1));
       }
+      const int nzm1 = nz - 1;
       auto live = [&](const BpRec &rec) {
         return vb < (int)(rec.w >> 16) && vb + span > (int)(rec.w & 0xffffu);
       };
@@ -745,19 +774,31 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
         const float a0 = __builtin_bit_cast(float, rec.y);
         const float a1 = __builtin_bit_cast(float, rec.z);
         const float ai = a0 * inv_dz;
+        // the S slots write S separate arrays: their first-slice updates
+        // go together, then (after the lanes' first slices are written)
+        // their second-slice updates
+        lds_f32 *c0p[S], *c1p[S];
+        float c0[S], c1[S], o0[S], o1[S];
 #pragma unroll
         for (int t = 0; t < S; ++t) {
-          const int k = med3_i32((int)(g.pzbz + ai * dzv[t]), 0, nz - 1);
-          float *c0p = acc + t * astride + 1 + k;
-          float *c1p = (float *)((char *)c0p + d4[t]);
-          const float az = zb[t][k] * izv[t];
+          const int k = med3_0s((int)(g.pzbz + ai * dzv[t]), nzm1);
+          c0p[t] = lds_ptr(k4_add_s(k, b0[t]));
+          c1p[t] = lds_ptr(k4_add(k, b1[t]));
+          const float az = *lds_ptr(k4_add(k, bz[t])) * izv[t];
           const float mz = min_sv(a1, az);
-          const float c0 = p[t] * (mz - a0), c1 = p[t] * (a1 - mz);
-          *c0p = *c0p + c0;
-          __builtin_amdgcn_wave_barrier();
-          *c1p = *c1p + c1;
-          __builtin_amdgcn_wave_barrier();
+          c0[t] = p[t] * (mz - a0);
+          c1[t] = p[t] * (a1 - mz);
         }
+#pragma unroll
+        for (int t = 0; t < S; ++t) o0[t] = *c0p[t];
+#pragma unroll
+        for (int t = 0; t < S; ++t) *c0p[t] = o0[t] + c0[t];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int t = 0; t < S; ++t) o1[t] = *c1p[t];
+#pragma unroll
+        for (int t = 0; t < S; ++t) *c1p[t] = o1[t] + c1[t];
+        __builtin_amdgcn_wave_barrier();
       };
       // kBpRecs records at a time: all their gathers are in flight before
       // the first update (the updates stay in record order)
