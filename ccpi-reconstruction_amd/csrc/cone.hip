@@ -845,6 +845,8 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
         const float a0 = __builtin_bit_cast(float, rec.y);
         const float a1 = __builtin_bit_cast(float, rec.z);
         const float ai = a0 * inv_dz;
+        float *c0p[S], *c1p[S];
+        float c0[S], c1[S];
 #pragma unroll
         for (int t = 0; t < S; ++t) {
           const int k = (int)(sl[t].kb + ai * sl[t].dzv) - sl[t].ks;
@@ -854,17 +856,32 @@ __global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
           // plane index to [0, nz]
           const float az = s_vz1[max(i0, i1)] * sl[t].izv;
           const float mz = min_sv(a1, az);
-          const float c0 = p[t] * (mz - a0), c1 = p[t] * (a1 - mz);
+          c0[t] = p[t] * (mz - a0);
+          c1[t] = p[t] * (a1 - mz);
           float *ac = acc + (ATOMIC ? 0 : t * astride) + 1;
-          if (ATOMIC) {
-            atomicAdd(&ac[i0], c0);
-            atomicAdd(&ac[i1], c1);
-          } else {
-            ac[i0] = ac[i0] + c0;
-            __builtin_amdgcn_wave_barrier();
-            ac[i1] = ac[i1] + c1;
-            __builtin_amdgcn_wave_barrier();
+          c0p[t] = ac + i0;
+          c1p[t] = ac + i1;
+        }
+        if (ATOMIC) {
+#pragma unroll
+          for (int t = 0; t < S; ++t) {
+            atomicAdd(c0p[t], c0[t]);
+            atomicAdd(c1p[t], c1[t]);
           }
+        } else {
+          // separate arrays per slot: first-slice updates together, then
+          // the second-slice updates
+          float o[S];
+#pragma unroll
+          for (int t = 0; t < S; ++t) o[t] = *c0p[t];
+#pragma unroll
+          for (int t = 0; t < S; ++t) *c0p[t] = o[t] + c0[t];
+          __builtin_amdgcn_wave_barrier();
+#pragma unroll
+          for (int t = 0; t < S; ++t) o[t] = *c1p[t];
+#pragma unroll
+          for (int t = 0; t < S; ++t) *c1p[t] = o[t] + c1[t];
+          __builtin_amdgcn_wave_barrier();
         }
       };
       // kBpRecs records at a time: all their gathers are in flight before
