@@ -693,8 +693,15 @@ __device__ inline BpSlot bp_slot(const ConeArgs &g, int v, int pass, bool one_se
 // of an array are trash for slices outside the volume.  ATOMIC (rows per
 // slice beyond the stride bound, or arrays beyond the LDS budget): S = 1,
 // one array, ds_add_f32.
+// At most 64 VGPRs (8 waves per SIMD) for strides 1 and 2: the only
+// spills this forces are in the per-row-block set-up, and the extra waves
+// hide the LDS read-add-write latency (S = 2 at 512^3: 98 -> 93 ms).  The
+// wider strides keep the registers their slots need.
+constexpr int bp_min_waves(int S) { return S <= 2 ? 8 : 1; }
+
 template <int S, bool ATOMIC>
-__global__ __launch_bounds__(64 * kBpWaves) void cone_bp_kernel(
+__global__ __launch_bounds__(64 * kBpWaves)
+__attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
     ConeArgs g, const BpCol *__restrict__ cols, const BpRec *__restrict__ recs,
     const float *__restrict__ dpix, float *__restrict__ vol) {
   static_assert(!ATOMIC || S == 1, "atomic accumulation uses one array");
