@@ -458,10 +458,10 @@ __global__ __launch_bounds__(kFpRows) void cone_nnz_kernel(
 // ---------------------------------------------------------------------------
 
 constexpr int kBpWaves = 4;
-// records per step of the BP sweep (512^3 cone, S = 2: 2 -> 118 ms, 4 -> 111,
-// 8 -> 104, 12 -> 134 as register pressure cuts occupancy)
+// records per step of the BP sweep (512^3 cone, S = 2, at most 64 VGPRs:
+// 4 -> 93.9 ms, 6 -> 92.7, 8 -> 93.0 (spills), 12 -> 119.9 (spills))
 template <int S>
-constexpr int bp_recs() { return S <= 2 ? 8 : 4; }
+constexpr int bp_recs() { return S <= 2 ? 6 : 4; }
 
 // Detector columns covered by the four corner lines of one voxel column
 // (c2D.cpp:837-901).
