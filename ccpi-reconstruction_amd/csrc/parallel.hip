@@ -30,7 +30,9 @@
 
 namespace ccpi {
 
-constexpr int kRows = 16;
+// slices per BP pass: 8 keeps the kernel at 91 VGPRs (5 waves per SIMD);
+// 16 needs 121 (4 waves) and was 5 % slower on the 64-problem workload
+constexpr int kRows = 8;
 
 __global__ __launch_bounds__(256) void par_fp_kernel(ParArgs g, int lr,
                                                      const float *__restrict__ vol,
