@@ -86,18 +86,21 @@ def cpu_baseline(geom, phantom_np, n_sample: int):
     }
 
 
-def batched_vs_plain(iters: int):
+def batched_vs_plain(iters: int, small: bool = False):
     """Config 3: 64 uneven parallel-beam CGLS problems, batched (one
     persistent launch per step, graph replay) vs the same problems one after
     another through the one-problem path.  Per-iteration time is
-    (T(iters) - T(0)) / iters for both, so set-up is excluded alike."""
+    (T(iters) - T(0)) / iters for both, so set-up is excluded alike.
+    ``small``: the same comparison on 256 problems too small to fill the
+    GPU one at a time (N in {64, 128}, S <= 4)."""
     import numpy as np
     import torch
 
     from ccpi.reconstruction import workloads as W
     from ccpi.reconstruction.projectors import CglsSession, cgls_batched, forward_project
 
-    geoms = W.batched_problems(64, seed=3)
+    geoms = (W.batched_problems(256, seed=5, sizes=(64, 128), slices=(1, 4), angles=(90, 180))
+             if small else W.batched_problems(64, seed=3))
     phs, bs = [], []
     for i, g in enumerate(geoms):
         ph = W.ellipsoid_phantom(g, 6, seed=300 + i, dens=(0.1, 1.0))
@@ -143,8 +146,11 @@ def batched_vs_plain(iters: int):
     for ss, _ in sess:
         ss.close()
     return {
-        "workload": "64 parallel-beam CGLS problems, N in {128..512}, S in [1, 32], "
-                    "{90,180,360} angles, width ceil(sqrt2 N), 1% Gaussian noise, seed 3",
+        "workload": ("256 small parallel-beam CGLS problems, N in {64, 128}, S in [1, 4], "
+                     "{90,180} angles, width ceil(sqrt2 N), 1% Gaussian noise, seed 5"
+                     if small else
+                     "64 parallel-beam CGLS problems, N in {128..512}, S in [1, 32], "
+                     "{90,180,360} angles, width ceil(sqrt2 N), 1% Gaussian noise, seed 3"),
         "iterations": iters,
         "batched_iter_s": 1.0 / per_b,
         "plain_iter_s": 1.0 / per_p,
@@ -303,6 +309,9 @@ def main() -> int:
         log("bench: config 1 (parallel CGLS) and config 2 SIRT / MLEM")
         extras = {"cfg1": parallel_cfg1(cpu=args.cpu_sample > 0),
                   "cfg2": cone_sirt_mlem(geom, b, 4)}
+        if args.batched_iters > 0:
+            log("bench: batched vs plain on small problems")
+            extras["batched_small"] = batched_vs_plain(args.batched_iters, small=True)
 
     base = None
     if args.cpu_sample > 0:

@@ -107,15 +107,16 @@ def gaussian_noise(sino, rel_sigma: float, seed: int):
     return (sino + sigma * torch.randn(sino.shape, generator=g, device=sino.device)).contiguous()
 
 
-def batched_problems(count: int = 64, seed: int = 3):
-    """Uneven parallel-beam problems: N in {128, 256, 384, 512},
-    S in [1, 32], angles in {90, 180, 360} over [0, pi), detector width
+def batched_problems(count: int = 64, seed: int = 3, sizes=(128, 256, 384, 512),
+                     slices=(1, 32), angles=(90, 180, 360)):
+    """Uneven parallel-beam problems: N in `sizes`, S in [slices[0],
+    slices[1]], angle counts from `angles` over [0, pi), detector width
     ceil(sqrt(2) N), 1 mm pixels (one geometry per problem)."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):
-        n = int(rng.choice([128, 256, 384, 512]))
-        s = int(rng.integers(1, 33))
-        na = int(rng.choice([90, 180, 360]))
+        n = int(rng.choice(list(sizes)))
+        s = int(rng.integers(slices[0], slices[1] + 1))
+        na = int(rng.choice(list(angles)))
         out.append(parallel_geometry(n, s, na, int(math.ceil(math.sqrt(2.0) * n))))
     return out
