@@ -208,6 +208,27 @@ def parallel_cfg1(cpu: bool):
     return out
 
 
+def cone_cgls_reg(geom, b, iters: int):
+    """Tikhonov- and TV-regularised CGLS (cgls.cpp:343-435) at the 512^3
+    geometry on the headline sinogram; per-iteration time as in
+    cone_sirt_mlem."""
+    import torch
+
+    from ccpi.reconstruction.projectors import TIKHONOV, TV, cgls_reg
+    out = {}
+    for name, kind, lam in (("tikhonov", TIKHONOV, 0.1), ("tv", TV, 1e-4)):
+        ts = []
+        for n in (1, iters):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            cgls_reg(geom, b, n, kind, lam)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        out[f"cgls_{name}_iter_s"] = (iters - 1) / max(ts[1] - ts[0], 1e-9)
+    out["workload"] = f"cone-beam regularised CGLS 512^3, {iters} iterations each"
+    return out
+
+
 def cone_sirt_mlem(geom, b, iters: int):
     """Config 2: cone-beam SIRT and MLEM at the 512^3 geometry, `iters`
     iterations each on the headline sinogram; per-iteration time is
@@ -308,7 +329,8 @@ def main() -> int:
     if args.extras:
         log("bench: config 1 (parallel CGLS) and config 2 SIRT / MLEM")
         extras = {"cfg1": parallel_cfg1(cpu=args.cpu_sample > 0),
-                  "cfg2": cone_sirt_mlem(geom, b, 4)}
+                  "cfg2": cone_sirt_mlem(geom, b, 4),
+                  "cgls_reg": cone_cgls_reg(geom, b, 3)}
         if args.batched_iters > 0:
             log("bench: batched vs plain on small problems")
             extras["batched_small"] = batched_vs_plain(args.batched_iters, small=True)

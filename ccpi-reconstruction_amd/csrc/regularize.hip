@@ -17,7 +17,7 @@
 // planes strided over blockIdx.y, so x / y neighbour reads hit the lines
 // that the neighbouring planes and rows fetch.  TV runs as two passes
 // through a 3-volume scratch array (D1, D2, D3); the terms are a few ms at
-// 512^3 against ~190 ms of projections per CGLS iteration.
+// 512^3 against ~180 ms of projections per CGLS iteration.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
