@@ -65,8 +65,10 @@ def tv_isotropic(vol: torch.Tensor, eps: float = 1e-8) -> torch.Tensor:
 
 
 def tikhonov(vol: torch.Tensor) -> torch.Tensor:
-    """0.5 ||L x||^2 with L the 6-neighbour Laplacian (zero boundary), the
-    3-D form of the stencil in Core/src/tikhonov.cpp:5-33."""
+    """0.5 ||L x||^2 with L the 6-neighbour 3-D Laplacian (zero boundary): a
+    smoothness penalty for gradient-based reconstruction.  The reference's
+    Tikhonov term (Core/src/tikhonov.cpp:4-32: the 4-neighbour xy stencil
+    with mirrored faces, used by cgls_tikhonov) is projectors.regularize."""
     p = torch.nn.functional.pad(vol[None, None], (1, 1, 1, 1, 1, 1))[0, 0]
     lap = (6.0 * vol - p[:-2, 1:-1, 1:-1] - p[2:, 1:-1, 1:-1] - p[1:-1, :-2, 1:-1]
            - p[1:-1, 2:, 1:-1] - p[1:-1, 1:-1, :-2] - p[1:-1, 1:-1, 2:])
