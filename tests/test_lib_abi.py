@@ -98,3 +98,27 @@ def test_cgls_reg_rejects_unknown_kind():
     dummy = C.c_void_p(16)
     assert L.ccpi_cgls_reg(None, dummy, dummy, 1, 5, 0.1, None, 0, None) == -1
     assert L.ccpi_cgls_reg(None, dummy, dummy, -1, 0, 0.1, None, 0, None) == -1
+
+
+def test_find_centre_rejects_bad_arguments():
+    """Null geometry, a parallel-beam geometry and a row outside the
+    detector are refused before any device work."""
+    L = _lib.lib()
+    y = C.c_double()
+    dummy = C.c_void_p(16)
+    assert L.ccpi_xtek_find_centre(None, dummy, 0, C.byref(y), None) == -1
+    ang = np.array([0.0, 0.5])
+    h = np.array([-1.0, 0.0, 1.0])
+    v = np.array([-0.5, 0.5])
+    g = _lib.CcpiGeometry(kind=_lib.CCPI_PARALLEL, n_angles=2, n_h=3, n_v=2,
+                          angles=_dp(ang), h_pixels=_dp(h), v_pixels=_dp(v),
+                          nx=2, ny=2, nz=2)
+    g.voxel_size[:] = [1.0, 1.0, 1.0]
+    g.origin[:] = [-1.0, -1.0, -1.0]
+    assert L.ccpi_xtek_find_centre(C.byref(g), dummy, 0, C.byref(y), None) == -1
+    assert b"cone" in L.ccpi_last_error()
+    g.kind = _lib.CCPI_CONE
+    g.source[:] = [-50.0, 0.0, 0.0]
+    g.detector_x = 50.0
+    assert L.ccpi_xtek_find_centre(C.byref(g), dummy, 5, C.byref(y), None) == -1
+    assert b"v_slice" in L.ccpi_last_error()
