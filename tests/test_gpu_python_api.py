@@ -160,3 +160,27 @@ def test_iterative_reconstructor_front_end():
     cor = r.getParameter("center_of_rotation")
     assert 40.0 < cor < 120.0
     assert vol.shape[2] == 8 and np.isfinite(vol).all()
+
+
+@pytest.mark.parametrize("algo", ["cgls", "sirt", "mlem"])
+def test_parallel_resolution_2_on_reference_fixture(algo):
+    """resolution (pixels per voxel) 2 on the reference's fixture: half as
+    many voxels per axis, two detector rows per slice (map type 2,
+    p2D.cpp:22-61), with the rotation centre of the data (80.0)."""
+    from ccpi.reconstruction import instruments as ins
+    from ccpi.reconstruction.parallelbeam import alg
+    sino = np.load(os.path.join(GOLDEN, "sino.npy")).astype(np.float32)
+    angles = np.load(os.path.join(GOLDEN, "angles.npy")).astype(np.float32)
+    vox = getattr(alg, algo)(sino, angles, np.double(80.0), 2, 3, 1, False)
+    setup = ins.diamond_setup(sino.shape, angles, 80.0, 2)
+    g = setup.geometry
+    b = np.zeros(g.sino_shape, np.float32)
+    off = setup.data_v_offset
+    b[:, :, off:off + sino.shape[1]] = -np.log(sino.transpose(0, 2, 1))
+    og = _og(g)
+    if algo == "cgls":
+        xo, _, _ = oracle.cgls(og, b, 3)
+    else:
+        xo = getattr(oracle, algo)(og, b, 3)
+    assert vox.shape == g.shape and g.shape[0] == 80 and g.shape[2] == 4
+    assert _rel(vox, np.maximum(xo, 0)) < 2e-4
