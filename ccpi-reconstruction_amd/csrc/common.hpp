@@ -75,10 +75,11 @@ using BpRec = uint4;
 struct ConeArgs {
   int na, nh, nv, nx, ny, nz;
   int mid;               // first detector row above the source
-  int bp_stride;         // BP row interleave (>= rows per slice)
+  int bp_stride;         // BP row distance of adjacent lanes
   int bp_unique;         // stride guarantees distinct slices per instruction
   int bp_main_safe;      // source inside the volume's z range
-  int bp_atomic;         // BP accumulates with LDS atomics (one array)
+  int bp_phased;         // BP accumulates in one array, in lane phases
+  int bp_phases;         // lane phases (lanes of one phase on distinct slices)
   int nvp;               // row stride of the BP's scaled sinogram (padded)
   float zb, zt;          // volume bottom / top relative to source z
   const float *alim;     // [nv] walk parameter where row v leaves the volume's

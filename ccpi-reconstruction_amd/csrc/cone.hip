@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -461,7 +462,7 @@ constexpr int kBpWaves = 4;
 // records per step of the BP sweep (512^3 cone, S = 2, at most 64 VGPRs:
 // 4 -> 93.9 ms, 6 -> 92.7, 8 -> 93.0 (spills), 12 -> 119.9 (spills))
 template <int S>
-constexpr int bp_recs() { return S <= 2 ? 6 : 4; }
+constexpr int bp_recs() { return S <= 2 ? 6 : (S <= 4 ? 4 : (S <= 8 ? 2 : 1)); }
 
 // Detector columns covered by the four corner lines of one voxel column
 // (c2D.cpp:837-901).
@@ -690,21 +691,140 @@ __device__ inline BpSlot bp_slot(const ConeArgs &g, int v, int pass, bool one_se
 // LDS array, so the slots' first-slice updates are plain read-add-writes
 // over distinct cells, issued together, followed by their second-slice
 // updates.  Cells 0 and nz + 1
-// of an array are trash for slices outside the volume.  ATOMIC (rows per
-// slice beyond the stride bound, or arrays beyond the LDS budget): S = 1,
-// one array, ds_add_f32.
+// of an array are trash for slices outside the volume.  PHASED (rows per
+// slice beyond 4, or arrays beyond the LDS budget): S = 4, 8 or 16 slots
+// and one array; the slots update one after another, each a read-add-write
+// round, and within a slot the lanes go in g.bp_phases phases (lane L in
+// phase L mod phases), whose lanes are phases * S >= rows per slice rows
+// apart, so on distinct slices.  (LDS float atomics on one array made the
+// kernel twice as slow as these rounds, and rows interleaved over passes
+// re-read each record's detector row once per pass.)
 // At most 64 VGPRs (8 waves per SIMD) for strides 1 and 2: the only
 // spills this forces are in the per-row-block set-up, and the extra waves
 // hide the LDS read-add-write latency (S = 2 at 512^3: 98 -> 93 ms).  The
 // wider strides keep the registers their slots need.
 constexpr int bp_min_waves(int S) { return S <= 2 ? 8 : 1; }
 
-template <int S, bool ATOMIC>
+// One block of the phased form: lane L holds rows vb + S*L + t, t < S;
+// every record's S rows are computed and written slot by slot into the one
+// array (cells acc1[-1] and acc1[nz] are trash).  MAIN: every row of the
+// block below nv is a main row (as in the slot form's main path); otherwise
+// each row takes its row set's form, pass by pass when the sets overlap (as
+// bp_slot).
+template <int S>
+__device__ void bp_phased_block(const ConeArgs &g, const BpRec *__restrict__ R, int n,
+                                const float *__restrict__ dpix, const float *s_vz1,
+                                float *acc1, int vb, int lane, int nph, int ph, float inv_dz,
+                                bool main, bool one_set, int min_v, int max_v) {
+  constexpr int span = 64 * S;
+  const int nz = g.nz, nv = g.nv, mid = g.mid;
+  const int vlane = vb + S * lane;
+  float dzv[S], izv[S];
+#pragma unroll
+  for (int t = 0; t < S; ++t) {
+    const int vv = min(vlane + t, nv - 1);
+    dzv[t] = g.dz[vv];
+    izv[t] = g.iz[vv];
+  }
+  const uint32_t lane_off = 4u * (uint32_t)(S * lane);
+  const int nvp4 = 4 * g.nvp;
+  auto live = [&](const BpRec &rec) {
+    return vb < (int)(rec.w >> 16) && vb + span > (int)(rec.w & 0xffffu);
+  };
+  auto gather = [&](const BpRec &rec, float *p) {
+    const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(dpix + (size_t)rec.x * g.nvp), (short)0, nvp4, kRsrcWord3);
+#pragma unroll
+    for (int t = 0; t < S; ++t)
+      p[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                          prow, (int)lane_off, 4 * (vb + t), 0));
+  };
+  auto update = [&](const BpRec &rec, const float *p, int pass, auto MAIN) {
+    const float a0 = __builtin_bit_cast(float, rec.y);
+    const float a1 = __builtin_bit_cast(float, rec.z);
+    const float ai = a0 * inv_dz;
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+      const int v = vlane + t;
+      int i0, i1, pl;
+      if constexpr (decltype(MAIN)::value) {
+        // padded rows (v >= nv, zero data) write the trash cell: with the
+        // last row's constants they would share its cell in this round
+        const int k = med3_0s((int)(g.pzbz + ai * dzv[t]), nz - 1);
+        const bool lower = v < mid, valid = v < nv;
+        i0 = valid ? k : -1;
+        i1 = valid ? (lower ? k - 1 : k + 1) : -1;
+        pl = lower ? k : k + 1;
+      } else {
+        const bool valid = v < nv;
+        int set;
+        bool member;
+        if (one_set) {
+          set = v < min_v ? 0 : (v < mid ? 1 : (v < max_v ? 2 : 3));
+          member = valid;
+        } else {
+          set = pass;
+          member = valid && ((set == 0)   ? (v < min_v)
+                             : (set == 1) ? (v >= min_v && v < mid)
+                             : (set == 2) ? (v >= mid && v < max_v)
+                                          : (v >= max_v));
+        }
+        const float kb = member ? (set == 0 ? g.pzbz1 : g.pzbz) : kBpOff;
+        const int k = (int)(kb + ai * dzv[t]) - (set == 0 ? 1 : 0);
+        i0 = med3_i32(k, -1, nz);
+        i1 = med3_i32(k + (set <= 1 ? -1 : 1), -1, nz);
+        pl = max(i0, i1);
+      }
+      const float az = s_vz1[pl] * izv[t];
+      const float mz = min_sv(a1, az);
+      const float c0 = p[t] * (mz - a0);
+      const float c1 = p[t] * (a1 - mz);
+      for (int q = 0; q < nph; ++q) {
+        if (ph == q) acc1[i0] += c0;
+        __builtin_amdgcn_wave_barrier();
+        if (ph == q) acc1[i1] += c1;
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+  };
+  auto sweep = [&](int pass, auto MAIN) {
+    constexpr int kBpRecs = bp_recs<S>();
+    int e = 0;
+    for (; e + kBpRecs <= n; e += kBpRecs) {
+      BpRec r[kBpRecs];
+      float p[kBpRecs][S];
+#pragma unroll
+      for (int u = 0; u < kBpRecs; ++u) r[u] = R[e + u];
+#pragma unroll
+      for (int u = 0; u < kBpRecs; ++u)
+        if (live(r[u])) gather(r[u], p[u]);
+#pragma unroll
+      for (int u = 0; u < kBpRecs; ++u)
+        if (live(r[u])) update(r[u], p[u], pass, MAIN);
+    }
+    for (; e < n; ++e) {
+      const BpRec r0 = R[e];
+      float p0[S];
+      if (live(r0)) {
+        gather(r0, p0);
+        update(r0, p0, pass, MAIN);
+      }
+    }
+  };
+  if (main) {
+    sweep(0, std::true_type{});
+  } else {
+    const int npass = one_set ? 1 : 4;
+    for (int pass = 0; pass < npass; ++pass) sweep(pass, std::false_type{});
+  }
+}
+
+template <int S, bool PHASED>
 __global__ __launch_bounds__(64 * kBpWaves)
 __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
     ConeArgs g, const BpCol *__restrict__ cols, const BpRec *__restrict__ recs,
     const float *__restrict__ dpix, float *__restrict__ vol) {
-  static_assert(!ATOMIC || S == 1, "atomic accumulation uses one array");
+  constexpr int narr = PHASED ? 1 : S;  // LDS arrays per wavefront
   extern __shared__ float smem[];
   const int nz = g.nz, nv = g.nv;
   // slice planes with plane 0 repeated in front: s_vzx[q + 1] = plane max(q, 0)
@@ -728,9 +848,9 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
   const int colid = ci * g.ny + cj;
   const BpCol C = cols[colid];
   if (C.n == 0) return;  // nothing crosses the column: it stays as it is
-  float *acc = smem + lds_pad4(nz + 2) + wave * S * astride;
+  float *acc = smem + lds_pad4(nz + 2) + wave * narr * astride;
   float *__restrict__ vcol = vol + (size_t)colid * nz;
-  for (int q = lane; q < S * astride; q += 64) acc[q] = 0.0f;
+  for (int q = lane; q < narr * astride; q += 64) acc[q] = 0.0f;
   __builtin_amdgcn_wave_barrier();
   for (int k = lane; k < nz; k += 64) acc[k + 1] = vcol[k];
   __builtin_amdgcn_wave_barrier();
@@ -738,6 +858,9 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
   const int min_v = C.min_v, max_v = C.max_v;
   const bool one_set = (min_v <= g.mid) && (max_v >= g.mid);
   const int npass = one_set ? 1 : 4;
+  // lane phases of the phased form (uniform count)
+  const int nph = PHASED ? g.bp_phases : 1;
+  const int ph = PHASED ? lane % nph : 0;
   constexpr int span = 64 * S;
   const uint32_t lane_off = 4u * (uint32_t)(S * lane);
   // 1 / voxel z size in a VGPR: a0 * inv_dz is then one v_mul with the
@@ -747,7 +870,12 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
   const BpRec *__restrict__ R = recs + C.off;
   const int nvp4 = 4 * g.nvp;
   for (int vb = 0; vb < nv; vb += span) {
-    if (!ATOMIC && one_set && g.bp_main_safe && vb >= min_v && vb + span <= max_v) {
+    if constexpr (PHASED) {
+      bp_phased_block<S>(g, R, C.n, dpix, s_vz1, acc + 1, vb, lane, nph, ph, inv_dz,
+                         one_set && g.bp_main_safe && vb >= min_v && min(vb + span, nv) <= max_v,
+                         one_set, min_v, max_v);
+      continue;
+    } else if (one_set && g.bp_main_safe && vb >= min_v && vb + span <= max_v) {
       // Every row of the block is a main row: its ray stays inside the
       // volume's z range for all of this column's rays (c2D.cpp:657-676), so
       // k is in [0, nz) (the clamp only guards float ties) and k +- 1 in
@@ -836,7 +964,8 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
     for (int pass = 0; pass < npass; ++pass) {
       BpSlot sl[S];
 #pragma unroll
-      for (int t = 0; t < S; ++t) sl[t] = bp_slot(g, vb + S * lane + t, pass, one_set, min_v, max_v);
+      for (int t = 0; t < S; ++t)
+        sl[t] = bp_slot(g, vb + S * lane + t, pass, one_set, min_v, max_v);
       auto live = [&](const BpRec &rec) {
         return vb < (int)(rec.w >> 16) && vb + span > (int)(rec.w & 0xffffu);
       };
@@ -865,17 +994,11 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
           const float mz = min_sv(a1, az);
           c0[t] = p[t] * (mz - a0);
           c1[t] = p[t] * (a1 - mz);
-          float *ac = acc + (ATOMIC ? 0 : t * astride) + 1;
+          float *ac = acc + t * astride + 1;
           c0p[t] = ac + i0;
           c1p[t] = ac + i1;
         }
-        if (ATOMIC) {
-#pragma unroll
-          for (int t = 0; t < S; ++t) {
-            atomicAdd(c0p[t], c0[t]);
-            atomicAdd(c1p[t], c1[t]);
-          }
-        } else {
+        {
           // separate arrays per slot: first-slice updates together, then
           // the second-slice updates
           float o[S];
@@ -922,7 +1045,7 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
   for (int k = lane; k < nz; k += 64) {
     float s = acc[k + 1];
 #pragma unroll
-    for (int t = 1; t < S; ++t) s += acc[t * astride + k + 1];
+    for (int t = 1; t < narr; ++t) s += acc[t * astride + k + 1];
     vcol[k] = s;
   }
 }
@@ -1059,7 +1182,7 @@ void cone_build_plan(ConePlan &P, hipStream_t st) {
     double pitch = 1e300;
     for (int v = 1; v < nv; v++) pitch = std::min(pitch, G.v_pix[v] - G.v_pix[v - 1]);
     const double rows_per_slice = G.vsize[2] / (amin * pitch);
-    const int want = std::max(1, (int)std::ceil(rows_per_slice * 1.05));
+    const int want = std::max(1, (int)std::ceil(std::min(rows_per_slice * 1.05, 1.0e6)));
     A.bp_stride = std::min(16, want);
     // lanes S rows apart are >= 1.05 slices apart: distinct slices
     A.bp_unique = want <= 16 ? 1 : 0;
@@ -1072,15 +1195,23 @@ void cone_build_plan(ConePlan &P, hipStream_t st) {
       uniform = uniform && std::fabs((G.v_pix[v] - G.v_pix[v - 1]) - st0) <= 1e-9 * std::fabs(st0);
     A.bp_main_safe =
         (uniform && sz >= G.origin[2] && sz < G.origin[2] + (double)nz * G.vsize[2]) ? 1 : 0;
-    // one LDS array per row phase while they fit two workgroups per CU
-    // beside each other; otherwise LDS atomics on one array
-    A.bp_atomic =
+    // one LDS array per row slot while they fit two workgroups per CU
+    // beside each other; otherwise one array, updated in lane phases
+    A.bp_phased =
         (A.bp_unique && A.bp_stride <= kBpMaxStride && bp_lds_bytes(nz, A.bp_stride) <= 64 * 1024)
             ? 0
             : 1;
-    CCPI_REQUIRE(bp_lds_bytes(nz, A.bp_atomic ? 1 : A.bp_stride) <= 160 * 1024, CCPI_E_LIMIT,
+    A.bp_phases = 1;
+    if (A.bp_phased) {
+      // 4, 8 or 16 slots; lanes of one phase are phases * S >= want rows
+      // apart
+      A.bp_stride = want <= 4 ? 4 : (want <= 8 ? 8 : 16);
+      A.bp_phases = std::min(64, (want + A.bp_stride - 1) / A.bp_stride);
+    }
+    CCPI_REQUIRE(bp_lds_bytes(nz, A.bp_phased ? 1 : A.bp_stride) <= 160 * 1024, CCPI_E_LIMIT,
                  "cone BP: nz too large for LDS");
-    const int span = 64 * (A.bp_atomic ? 1 : A.bp_stride);
+    // rows of one BP block: 64 lanes bp_stride rows apart (both forms)
+    const int span = 64 * A.bp_stride;
     A.nvp = (nv + span - 1) / span * span;
     CCPI_REQUIRE(nv < 65536, CCPI_E_LIMIT, "cone beam: at most 65535 detector rows");
   }
@@ -1216,18 +1347,22 @@ void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, hipStream_
   const ConeArgs &A = P.args;
   // 32 x 32 column tiles of 32 * 32 / kBpWaves workgroups (kernel mapping)
   const int blocks = ((A.nx + 31) / 32) * ((A.ny + 31) / 32) * (32 * 32 / kBpWaves);
-  const int S = A.bp_atomic ? 1 : A.bp_stride;
-  const size_t lds = bp_lds_bytes(A.nz, S);
+  const int narr = A.bp_phased ? 1 : A.bp_stride;  // LDS arrays per wavefront
+  const size_t lds = bp_lds_bytes(A.nz, narr);
   const BpCol *bc = P.bpcol.as<BpCol>();
   const BpRec *br = P.bprec.as<BpRec>();
   const dim3 grid(blocks), block(64 * kBpWaves);
-  if (A.bp_atomic)
-    hipLaunchKernelGGL((cone_bp_kernel<1, true>), grid, block, lds, st, A, bc, br, dpix, vol);
-  else if (S == 1)
+  if (A.bp_phased && A.bp_stride == 4)
+    hipLaunchKernelGGL((cone_bp_kernel<4, true>), grid, block, lds, st, A, bc, br, dpix, vol);
+  else if (A.bp_phased && A.bp_stride == 8)
+    hipLaunchKernelGGL((cone_bp_kernel<8, true>), grid, block, lds, st, A, bc, br, dpix, vol);
+  else if (A.bp_phased)
+    hipLaunchKernelGGL((cone_bp_kernel<16, true>), grid, block, lds, st, A, bc, br, dpix, vol);
+  else if (narr == 1)
     hipLaunchKernelGGL((cone_bp_kernel<1, false>), grid, block, lds, st, A, bc, br, dpix, vol);
-  else if (S == 2)
+  else if (narr == 2)
     hipLaunchKernelGGL((cone_bp_kernel<2, false>), grid, block, lds, st, A, bc, br, dpix, vol);
-  else if (S == 3)
+  else if (narr == 3)
     hipLaunchKernelGGL((cone_bp_kernel<3, false>), grid, block, lds, st, A, bc, br, dpix, vol);
   else
     hipLaunchKernelGGL((cone_bp_kernel<4, false>), grid, block, lds, st, A, bc, br, dpix, vol);

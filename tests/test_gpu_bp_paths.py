@@ -1,14 +1,20 @@
 """Every cone backprojection kernel instance against the oracle (GPU test).
 
 The cone BP picks its row stride S from the largest number of detector rows
-per slice (at least 1.05 slices between the rows of one slot) and falls back
-to LDS atomics on one array when S would exceed 4 or the S arrays would not
-fit 64 KB per workgroup.  These geometries select S = 1, 2 (the default test
-cases), 3, 4 and both atomic fallbacks; each is checked for FP and BP parity
-(1e-6 / 1e-5 of the maximum, as in test_gpu_parity) and the adjoint
-identity.  ``_path`` restates the host's choice so that a geometry that no
-longer reaches its intended kernel fails here instead of silently testing
-another one.
+per slice (at least 1.05 slices between the rows of one slot) and uses S
+LDS arrays per wavefront while S <= 4 and they fit 64 KB per workgroup.
+Otherwise it takes the phased form: S = 4, 8 or 16 slots on one array,
+written slot after slot, and within a slot the lanes in ceil(want / S)
+phases.  These geometries select S = 1, 2 (the default test cases), 3, 4,
+and the phased form with S = 8 (fewer than 64 rows; a padded block of main
+rows, every ray inside the volume's z range; two blocks, the second padded,
+with rows outside it), with
+S = 16 in one and in two phases (wide cones: the volume's corners close to
+the source), and with S = 4 (arrays beyond the LDS budget).  Each is checked
+for FP and BP parity (1e-6 / 1e-5 of the maximum, as in test_gpu_parity)
+and the adjoint identity.  ``_path`` restates the host's choice so that a
+geometry that no longer reaches its intended kernel fails here instead of
+silently testing another one.
 """
 
 import math
@@ -23,22 +29,33 @@ from geometries import CB, centred, ellipsoid_phantom, oracle_geom
 
 pytestmark = pytest.mark.gpu
 
+A12 = np.linspace(0, 2 * np.pi, 12, endpoint=False) + 0.1
+
 CASES = {
     "s1": (CB((10, 10, 8), (1, 1, 1), np.linspace(0, 2 * np.pi, 12, endpoint=False) + 0.1,
-              centred(20, 1.6), centred(8, 3.0), 40.0, 80.0), (1, False)),
+              centred(20, 1.6), centred(8, 3.0), 40.0, 80.0), (1, False, 1)),
     "s3": (CB((10, 10, 6), (1, 1, 1), np.linspace(0, 2 * np.pi, 12, endpoint=False) + 0.1,
-              centred(20, 1.6), centred(20, 0.95), 40.0, 80.0), (3, False)),
+              centred(20, 1.6), centred(20, 0.95), 40.0, 80.0), (3, False, 1)),
     "s4": (CB((10, 10, 6), (1, 1, 1), np.linspace(0, 2 * np.pi, 12, endpoint=False) + 0.1,
-              centred(20, 1.6), centred(24, 0.75), 40.0, 80.0), (4, False)),
-    "atomic_stride": (CB((8, 8, 4), (1, 1, 1), np.linspace(0, 2 * np.pi, 10, endpoint=False),
-                         centred(16, 1.6), centred(40, 0.4), 40.0, 80.0), (7, True)),
-    "atomic_lds": (CB((4, 4, 1500), (1, 1, 0.05), np.linspace(0, 2 * np.pi, 8, endpoint=False),
-                      centred(10, 1.2), centred(3000, 0.05), 40.0, 80.0), (3, True)),
+              centred(20, 1.6), centred(24, 0.75), 40.0, 80.0), (4, False, 1)),
+    "phased_s8_small": (CB((8, 8, 4), (1, 1, 1), np.linspace(0, 2 * np.pi, 10, endpoint=False),
+                           centred(16, 1.6), centred(40, 0.4), 40.0, 80.0), (8, True, 1)),
+    "phased_s8_main": (CB((12, 12, 200), (1, 1, 1), A12, centred(20, 1.6), centred(460, 0.5),
+                          20.0, 40.0), (8, True, 1)),
+    "phased_s8_blocks": (CB((12, 12, 160), (1, 1, 0.25), A12, centred(20, 1.6),
+                            centred(700, 0.17), 20.0, 40.0), (8, True, 1)),
+    "phased_s16": (CB((12, 12, 40), (1, 1, 1), A12, centred(20, 1.6), centred(700, 0.25),
+                      24.0, 48.0), (16, True, 1)),
+    "phased_s16_2ph": (CB((12, 12, 40), (1, 1, 1), A12, centred(20, 1.6), centred(700, 0.2),
+                          24.0, 48.0), (16, True, 2)),
+    "phased_s4_lds": (CB((4, 4, 1500), (1, 1, 0.05), np.linspace(0, 2 * np.pi, 8, endpoint=False),
+                         centred(10, 1.2), centred(3000, 0.05), 40.0, 80.0), (4, True, 1)),
 }
 
 
 def _path(g):
-    """(row stride, atomic) as the host chooses them (cone.hip plan build)."""
+    """(row stride, phased, phases) as the host chooses them (cone.hip plan
+    build)."""
     nx, ny, nz = g.shape
     vs, o = g.vsize, g.origin
     R = max(math.hypot(o[0] + cx * nx * vs[0], o[1] + cy * ny * vs[1])
@@ -49,7 +66,11 @@ def _path(g):
     s = min(16, want)
     pad = (nz + 2 + 3) & ~3
     lds = 4 * (pad + 4 * s * pad)
-    return s, not (want <= 16 and s <= 4 and lds <= 64 * 1024)
+    phased = not (want <= 16 and s <= 4 and lds <= 64 * 1024)
+    if not phased:
+        return s, False, 1
+    s = 4 if want <= 4 else (8 if want <= 8 else 16)
+    return s, True, min(64, -(-want // s))
 
 
 def _geom(g):
@@ -69,7 +90,8 @@ def test_bp_path_parity(name):
     assert _path(g) == want_path
     og = oracle_geom(g)
     x = ellipsoid_phantom(g.shape, g.vsize, g.origin, 4, seed=7).astype(np.float32)
-    print(f"\ncone BP path {name}: S={want_path[0]} atomic={want_path[1]}", flush=True)
+    print(f"\ncone BP path {name}: S={want_path[0]} phased={want_path[1]} "
+          f"phases={want_path[2]}", flush=True)
     y = forward_project(_geom(g), torch.from_numpy(x).cuda()).cpu().numpy()
     assert _rel(y, og.fp(x)) < 1e-6
     s = np.random.default_rng(8).random(g.sino_shape).astype(np.float32)
