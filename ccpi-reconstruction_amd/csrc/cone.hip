@@ -726,18 +726,23 @@ __device__ void bp_phased_block(const ConeArgs &g, const BpRec *__restrict__ R, 
     dzv[t] = g.dz[vv];
     izv[t] = g.iz[vv];
   }
-  const uint32_t lane_off = 4u * (uint32_t)(S * lane);
-  const int nvp4 = 4 * g.nvp;
+  static_assert(S % 4 == 0, "the phased form loads 4 rows at a time");
   auto live = [&](const BpRec &rec) {
     return vb < (int)(rec.w >> 16) && vb + span > (int)(rec.w & 0xffffu);
   };
   auto gather = [&](const BpRec &rec, float *p) {
-    const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)(dpix + (size_t)rec.x * g.nvp), (short)0, nvp4, kRsrcWord3);
+    // rows vlane .. vlane + S - 1 of the record's padded row, 16 bytes per
+    // load (nvp is a multiple of 64 * S, so every row is in range; one
+    // dword per load made the wide-cone BP 1.8x slower)
+    const float4 *src = (const float4 *)(dpix + (size_t)rec.x * g.nvp + vlane);
 #pragma unroll
-    for (int t = 0; t < S; ++t)
-      p[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                          prow, (int)lane_off, 4 * (vb + t), 0));
+    for (int j = 0; j < S / 4; ++j) {
+      const float4 q = src[j];
+      p[4 * j] = q.x;
+      p[4 * j + 1] = q.y;
+      p[4 * j + 2] = q.z;
+      p[4 * j + 3] = q.w;
+    }
   };
   auto update = [&](const BpRec &rec, const float *p, int pass, auto MAIN) {
     const float a0 = __builtin_bit_cast(float, rec.y);
