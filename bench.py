@@ -250,6 +250,55 @@ def cone_sirt_mlem(geom, b, iters: int):
     return out
 
 
+def cone_wide_bp(reps: int = 3):
+    """The headline scan read through the XTek route (xtek_setup: 720 x 768^2
+    detector of 1 mm pixels, SOD 500 mm, SDD 1000 mm), whose volume rule
+    (xtek.cpp:760-798) gives 768^3 voxels of 0.811 mm with corners 60 mm
+    from the source: up to 13.5 detector rows per slice, so the BP takes its
+    phased form.  Best of `reps` BPs / FPs of a seeded random sinogram /
+    volume (HIP events on the launch stream), and the BP's algorithmic
+    bytes as for the roofline."""
+    import numpy as np
+    import torch
+
+    from ccpi.reconstruction import instruments as ins
+    from ccpi.reconstruction import projectors as P
+    na, nh, nv = 720, 768, 768
+    ang = np.linspace(0, 360, na, endpoint=False).astype(np.float32)
+    g = ins.xtek_setup((na, nh, nv), ang, 1, -500.0, 500.0, 1.0, 1.0, 128.0).geometry
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    s = torch.rand(g.sino_shape, device="cuda", generator=gen)
+    x = torch.rand(g.shape, device="cuda", generator=gen)
+    nnz = P.projector_nnz(g)
+
+    def timed(f):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    vol = torch.zeros(g.shape, device="cuda")
+    sino = torch.zeros(g.sino_shape, device="cuda")
+    bps, fps = [], []
+    for _ in range(reps):
+        vol.zero_()
+        bps.append(timed(lambda: P.backward_project(g, s, vol)))
+        sino.zero_()
+        fps.append(timed(lambda: P.forward_project(g, x, sino)))
+    bp, fp = min(bps), min(fps)
+    nvox = int(np.prod(g.shape))
+    nray = int(np.prod(g.sino_shape))
+    bytes_alg = 4 * nnz + 4 * (nray + nvox)
+    gbs = bytes_alg / (bp * 1e-3) / 1e9
+    return {"workload": "cone-beam BP, XTek volume rule: 768^3 of 0.811 mm, 720 x 768^2 "
+                        "detector (1 mm), SOD 500 mm, SDD 1000 mm",
+            "bp_ms": bp, "fp_ms": fp, "nnz": int(nnz), "bp_gbs": gbs,
+            "bp_roofline_frac": gbs / HBM_PEAK_GBS}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -334,6 +383,8 @@ def main() -> int:
         if args.batched_iters > 0:
             log("bench: batched vs plain on small problems")
             extras["batched_small"] = batched_vs_plain(args.batched_iters, small=True)
+        log("bench: wide-cone BP (XTek volume rule)")
+        extras["cone_wide_bp"] = cone_wide_bp()
 
     base = None
     if args.cpu_sample > 0:
