@@ -18,11 +18,14 @@
 //            detector rows, so the xy walk is uniform (scalar loads of the
 //            entry list) and each segment's voxel column reads are
 //            near-contiguous in z across the wave.
-//   BP       one wavefront per voxel column: lanes find the column's
-//            (angle, h) rays in parallel (double), compact them into LDS,
-//            then sweep rows and accumulate into an LDS copy of the column
-//            with ds_add_f32.  One wave owns a column, so the BP needs no
-//            global atomics and is deterministic.
+//   BP       one wavefront per voxel column: the plan lists the column's
+//            (angle, h) rays (double footprints); the list is swept once
+//            per block of 64*S detector rows, S consecutive rows per lane,
+//            and each row's two slice parts go into an LDS copy of the
+//            column by plain read-add-writes of distinct cells (one LDS
+//            array per row slot, or, past 4 rows per slice, one array
+//            written slot by slot in lane phases).  One wave owns a column,
+//            so the BP needs no global atomics and is deterministic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1075,7 +1078,8 @@ __global__ void cone_scale_kernel(const float *__restrict__ pix,
 // host side
 // ---------------------------------------------------------------------------
 
-// largest row stride with read-add-write accumulation (one kernel each)
+// largest row stride of the slot form (one LDS array per slot, one kernel
+// each); beyond it the phased form
 constexpr int kBpMaxStride = 4;
 
 // LDS of one BP workgroup: the slice planes and `narr` column arrays per
