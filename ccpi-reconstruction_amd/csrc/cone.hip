@@ -1213,9 +1213,16 @@ void cone_build_plan(ConePlan &P, hipStream_t st) {
     A.bp_phases = 1;
     if (A.bp_phased) {
       // 4, 8 or 16 slots; lanes of one phase are phases * S >= want rows
-      // apart
-      A.bp_stride = want <= 4 ? 4 : (want <= 8 ? 8 : 16);
-      A.bp_phases = std::min(64, (want + A.bp_stride - 1) / A.bp_stride);
+      // apart.  Past 4 rows per slice, the S of least (padded rows) x (1 +
+      // 0.11 per extra phase): measured on wide cones, 768 rows: S = 16
+      // 681 ms, S = 8 in 2 phases 756 ms; 384 rows: 83.6 and 47.9 ms.
+      auto phases = [&](int S) { return std::min(64, (want + S - 1) / S); };
+      auto cost = [&](int S) {
+        const double rows = (double)((nv + 64 * S - 1) / (64 * S)) * 64.0 * S;
+        return rows * (1.0 + 0.11 * (phases(S) - 1));
+      };
+      A.bp_stride = want <= 4 ? 4 : (cost(8) <= cost(16) ? 8 : 16);
+      A.bp_phases = phases(A.bp_stride);
     }
     CCPI_REQUIRE(bp_lds_bytes(nz, A.bp_phased ? 1 : A.bp_stride) <= 160 * 1024, CCPI_E_LIMIT,
                  "cone BP: nz too large for LDS");

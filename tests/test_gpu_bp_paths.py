@@ -5,16 +5,17 @@ per slice (at least 1.05 slices between the rows of one slot) and uses S
 LDS arrays per wavefront while S <= 4 and they fit 64 KB per workgroup.
 Otherwise it takes the phased form: S = 4, 8 or 16 slots on one array,
 written slot after slot, and within a slot the lanes in ceil(want / S)
-phases.  These geometries select S = 1, 2 (the default test cases), 3, 4,
-and the phased form with S = 8 (fewer than 64 rows; a padded block of main
-rows, every ray inside the volume's z range; two blocks, the second padded,
-with rows outside it), with
-S = 16 in one and in two phases (wide cones: the volume's corners close to
-the source), and with S = 4 (arrays beyond the LDS budget).  Each is checked
-for FP and BP parity (1e-6 / 1e-5 of the maximum, as in test_gpu_parity)
-and the adjoint identity.  ``_path`` restates the host's choice so that a
-geometry that no longer reaches its intended kernel fails here instead of
-silently testing another one.
+phases (past 4 rows per slice, S = 8 or 16 by the least padded rows x
+(1 + 0.11 per extra phase)).  These geometries select S = 1, 2 (the
+default test cases), 3, 4, and the phased form with S = 8 (fewer than 64
+rows; a padded block of main rows, every ray inside the volume's z range;
+two blocks, the second padded, with rows outside it), S = 8 in two phases
+(300 rows), S = 16 in one and in two phases (wide cones: the volume's
+corners close to the source), and S = 4 (arrays beyond the LDS budget).
+Each is checked for FP and BP parity (1e-6 / 1e-5 of the maximum, as in
+test_gpu_parity) and the adjoint identity.  ``_path`` restates the host's
+choice so that a geometry that no longer reaches its intended kernel fails
+here instead of silently testing another one.
 """
 
 import math
@@ -46,6 +47,8 @@ CASES = {
                             centred(700, 0.17), 20.0, 40.0), (8, True, 1)),
     "phased_s16": (CB((12, 12, 40), (1, 1, 1), A12, centred(20, 1.6), centred(700, 0.25),
                       24.0, 48.0), (16, True, 1)),
+    "phased_s8_2ph": (CB((12, 12, 40), (1, 1, 1), A12, centred(20, 1.6), centred(300, 0.25),
+                         24.0, 48.0), (8, True, 2)),
     "phased_s16_2ph": (CB((12, 12, 40), (1, 1, 1), A12, centred(20, 1.6), centred(700, 0.2),
                           24.0, 48.0), (16, True, 2)),
     "phased_s4_lds": (CB((4, 4, 1500), (1, 1, 0.05), np.linspace(0, 2 * np.pi, 8, endpoint=False),
@@ -69,8 +72,16 @@ def _path(g):
     phased = not (want <= 16 and s <= 4 and lds <= 64 * 1024)
     if not phased:
         return s, False, 1
-    s = 4 if want <= 4 else (8 if want <= 8 else 16)
-    return s, True, min(64, -(-want // s))
+    nv = len(g.v_pix)
+
+    def phases(s):
+        return min(64, -(-want // s))
+
+    def cost(s):
+        return -(-nv // (64 * s)) * 64 * s * (1.0 + 0.11 * (phases(s) - 1))
+
+    s = 4 if want <= 4 else (8 if cost(8) <= cost(16) else 16)
+    return s, True, phases(s)
 
 
 def _geom(g):

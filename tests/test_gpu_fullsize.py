@@ -16,7 +16,7 @@ The same properties hold for the BP's phased form (more than 4 detector rows
 per slice) on a wide cone at size: a 360 x 384^2 scan of 2 mm pixels (SOD
 500, SDD 1000) through the XTek volume rule (instruments.xtek_setup,
 xtek.cpp:760-798) gives 384^3 voxels of 1.62 mm whose corners come within
-60 mm of the source, 14.5 rows per slice at most: S = 16.
+60 mm of the source, 14.5 rows per slice at most: S = 8 in two phases.
 """
 
 import numpy as np
