@@ -100,6 +100,12 @@ void Geometry::validate() const {
                  "source inside the volume's rotation circle");
     CCPI_REQUIRE(det_x > source[0], CCPI_E_GEOMETRY,
                  "detector must be on the far side of the source");
+    // the slice logic of f2D / b2D (c2D.cpp:17-191, 657-749) takes the
+    // source within the volume's z extent: half a slice beyond it the
+    // reference's sums are undefined (ray sums of order 1e35, varying run
+    // to run, on an 8x8x4 volume whose exact chords are at most 10)
+    CCPI_REQUIRE(source[2] >= origin[2] && source[2] <= origin[2] + (double)nz * vsize[2],
+                 CCPI_E_GEOMETRY, "source outside the volume's z range");
   }
 }
 

@@ -7,11 +7,13 @@ Otherwise it takes the phased form: S = 4, 8 or 16 slots on one array,
 written slot after slot, and within a slot the lanes in ceil(want / S)
 phases (past 4 rows per slice, S = 8 or 16 by the least padded rows x
 (1 + 0.11 per extra phase)).  These geometries select S = 1, 2 (the
-default test cases), 3, 4, and the phased form with S = 8 (fewer than 64
-rows; a padded block of main rows, every ray inside the volume's z range;
-two blocks, the second padded, with rows outside it), S = 8 in two phases
-(300 rows), S = 16 in one and in two phases (wide cones: the volume's
-corners close to the source), and S = 4 (arrays beyond the LDS budget).
+default test cases), 3, 4, and the phased form with:
+  * S = 8: fewer than 64 rows, with the source at mid-height and near the
+    volume's top; a padded block of main rows (every ray inside the
+    volume's z range); two blocks, the second padded, with rows outside it;
+  * S = 8 in two phases (300 rows), S = 16 in one and in two phases (wide
+    cones: the volume's corners close to the source);
+  * S = 4 (arrays beyond the LDS budget).
 Each is checked for FP and BP parity (1e-6 / 1e-5 of the maximum, as in
 test_gpu_parity) and the adjoint identity.  ``_path`` restates the host's
 choice so that a geometry that no longer reaches its intended kernel fails
@@ -41,6 +43,9 @@ CASES = {
               centred(20, 1.6), centred(24, 0.75), 40.0, 80.0), (4, False, 1)),
     "phased_s8_small": (CB((8, 8, 4), (1, 1, 1), np.linspace(0, 2 * np.pi, 10, endpoint=False),
                            centred(16, 1.6), centred(40, 0.4), 40.0, 80.0), (8, True, 1)),
+    "phased_s8_offz": (CB((8, 8, 4), (1, 1, 1), np.linspace(0, 2 * np.pi, 10, endpoint=False),
+                          centred(16, 1.6), centred(40, 0.4), 40.0, 80.0, source_z=1.7),
+                       (8, True, 1)),
     "phased_s8_main": (CB((12, 12, 200), (1, 1, 1), A12, centred(20, 1.6), centred(460, 0.5),
                           20.0, 40.0), (8, True, 1)),
     "phased_s8_blocks": (CB((12, 12, 160), (1, 1, 0.25), A12, centred(20, 1.6),
@@ -54,6 +59,12 @@ CASES = {
     "phased_s4_lds": (CB((4, 4, 1500), (1, 1, 0.05), np.linspace(0, 2 * np.pi, 8, endpoint=False),
                          centred(10, 1.2), centred(3000, 0.05), 40.0, 80.0), (4, True, 1)),
 }
+
+
+# With the source inside the volume's top slice the reference's b2D is not
+# the transpose of its f2D (the oracle's own <Ax, y> and <x, A^T y> differ
+# by 31 % for this case; test_oracle_pins maps the heights): parity only.
+NOT_ADJOINT = {"phased_s8_offz"}
 
 
 def _path(g):
@@ -108,6 +119,8 @@ def test_bp_path_parity(name):
     s = np.random.default_rng(8).random(g.sino_shape).astype(np.float32)
     b = backward_project(_geom(g), torch.from_numpy(s).cuda()).cpu().numpy()
     assert _rel(b, og.bp(s)) < 1e-5
+    if name in NOT_ADJOINT:
+        return
     lhs = float(np.vdot(y.astype(np.float64), s))
     rhs = float(np.vdot(x.astype(np.float64), b))
     assert abs(lhs - rhs) < 1e-5 * abs(lhs)

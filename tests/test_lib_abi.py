@@ -62,7 +62,10 @@ def test_parallel_geometry_rejected(case):
 
 def test_cone_geometry_rejected():
     """A detector row at the source height (the reference's 'Oops',
-    c2D.cpp:487-492) and nz < 2 (b2D divides by nz - 1, c2D.cpp:808)."""
+    c2D.cpp:487-492), nz < 2 (b2D divides by nz - 1, c2D.cpp:808), a source
+    inside the rotation circle, and a source outside the volume's z range,
+    where the oracle (the reference's f2D) returns sums off the exact chords
+    by orders of magnitude."""
     L = _lib.lib()
     h = np.array([-1.0, 1.0])
     ang = np.array([0.0])
@@ -78,6 +81,11 @@ def test_cone_geometry_rejected():
     # source inside the volume's rotation circle
     assert L.ccpi_cb_f2D(-2.0, 0.0, 0.0, 50.0, _dp(h), _dp(v), _dp(ang), dummy, dummy,
                          1, 2, 2, _dp(o), _dp(s), 4, 4, 4, None) == -2
+    # source above the volume's z range [-2, 2] (the reference's slice
+    # logic has no defined result there)
+    assert L.ccpi_cb_b2D(-50.0, 0.0, 2.5, 50.0, _dp(h), _dp(v), _dp(ang), dummy, dummy,
+                         1, 2, 2, _dp(o), _dp(s), 4, 4, 4, None) == -2
+    assert b"z range" in L.ccpi_last_error()
 
 
 def test_regularize_rejects_bad_arguments():

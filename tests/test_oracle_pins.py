@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 import oracle
-from geometries import ALL_CB, ALL_PB, dense, oracle_geom, centred, PB
+from geometries import ALL_CB, ALL_PB, CB, PB, centred, dense, oracle_geom
 
 # float32 weights come from float alpha differences; relative to the
 # output scale the model differs from exact geometry by a few 1e-6.
@@ -173,3 +173,37 @@ def test_mlem_preserves_total_counts_and_converges():
         assert abs((A @ x.ravel()).sum() - b.sum()) < 1e-10 * b.sum()
     x = oracle.mlem(og, b.reshape(g.sino_shape), 3000, dtype=np.float64)
     assert _rel(x.ravel(), xt) < 1e-3
+
+
+def _offz_case(sz):
+    return CB((8, 8, 4), (1.0, 1.0, 1.0), np.linspace(0, 2 * np.pi, 10, endpoint=False),
+              centred(16, 1.6), centred(40, 0.4), 40.0, 80.0, source_z=sz)
+
+
+@pytest.mark.parametrize("sz", [0.3, 0.99, -0.99, -1.0, 1.0, 1.7, -1.7])
+def test_source_height_fp_exact_bp_transpose_outside_end_slices(sz):
+    """A source off the volume's mid-height (z range [-2, 2], 1 mm slices):
+    A 1 is the exact ray-box chord (tests/bruteforce.py) for every height,
+    and b2D is the transpose of f2D for heights in [-1, 1), the source
+    outside the end slices.  From the top slice's lower plane up, or inside
+    the bottom slice, the reference's b2D departs from the transpose
+    (recorded here: 15 % at 1.0, a factor 12 at 1.3; the device BP
+    reproduces it, tests/test_gpu_bp_paths.py)."""
+    import bruteforce as bf
+    g = _offz_case(sz)
+    og = oracle_geom(g)
+    y1 = og.fp(np.ones(g.shape))
+    lo = tuple(g.origin)
+    hi = tuple(g.origin[q] + g.shape[q] * g.vsize[q] for q in range(3))
+    for a in range(len(g.angles)):
+        want = bf.cone_box_lengths(g.source, g.det_x, g.angles[a], g.h_pix, g.v_pix, [(lo, hi)])
+        assert np.abs(y1[a] - want).max() <= 1e-9 * want.max()
+    rng = np.random.default_rng(11)
+    x = rng.random(g.shape)
+    y = rng.random(g.sino_shape)
+    lhs = np.vdot(og.fp(x), y)
+    rhs = np.vdot(x, og.bp(y))
+    if -1.0 <= sz < 1.0:
+        assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
+    else:
+        assert abs(lhs - rhs) > 0.1 * abs(lhs)
