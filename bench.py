@@ -310,6 +310,8 @@ def main() -> int:
                     help="CGLS iterations of the batched-vs-plain comparison (0 skips it)")
     ap.add_argument("--extras", type=int, default=1,
                     help="also measure config 1 (parallel CGLS) and config 2 SIRT/MLEM")
+    ap.add_argument("--wide", type=int, default=1,
+                    help="with --extras: also time the wide-cone BP (0 disables it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device to device (the known-size kernel that "
                          "scales FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -383,8 +385,9 @@ def main() -> int:
         if args.batched_iters > 0:
             log("bench: batched vs plain on small problems")
             extras["batched_small"] = batched_vs_plain(args.batched_iters, small=True)
-        log("bench: wide-cone BP (XTek volume rule)")
-        extras["cone_wide_bp"] = cone_wide_bp()
+        if args.wide:
+            log("bench: wide-cone BP (XTek volume rule)")
+            extras["cone_wide_bp"] = cone_wide_bp()
 
     base = None
     if args.cpu_sample > 0:

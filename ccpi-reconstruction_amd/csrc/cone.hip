@@ -462,6 +462,11 @@ __global__ __launch_bounds__(kFpRows) void cone_nnz_kernel(
 // ---------------------------------------------------------------------------
 
 constexpr int kBpWaves = 4;
+// BP column tile: consecutive workgroups of one XCD take kBpTile x kBpTile
+// adjacent columns, which share most of their rays' detector rows in L2
+// (BP at 512^3, tiles of 32 / 16 / 8: 92.0 / 90.5 / 90.6 ms; wide cone
+// 768^3: 684 / 674 / 663 ms; 64: 97.6 / 691 ms)
+constexpr int kBpTile = 8;
 // records per step of the BP sweep (512^3 cone, S = 2, at most 64 VGPRs:
 // 4 -> 93.9 ms, 6 -> 92.7, 8 -> 93.0 (spills), 12 -> 119.9 (spills))
 template <int S>
@@ -844,10 +849,10 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
   // the wave index is uniform: keep the column's loads scalar
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
-  // logical block -> 32 x 32 column tile (one XCD's resident set), then
-  // kBpWaves adjacent columns of one tile row
+  // logical block -> kBpTile x kBpTile column tile, then kBpWaves adjacent
+  // columns of one tile row
   const int lb = xcd_block(blockIdx.x, gridDim.x);
-  constexpr int kTile = 32, kTileBlocks = kTile * kTile / kBpWaves;
+  constexpr int kTile = kBpTile, kTileBlocks = kTile * kTile / kBpWaves;
   const int tile = lb / kTileBlocks, within = lb - tile * kTileBlocks;
   const int ntj = (g.ny + kTile - 1) / kTile;
   const int ci = (tile / ntj) * kTile + within / (kTile / kBpWaves);
@@ -1361,8 +1366,10 @@ void cone_scale(const ConePlan &P, const float *pix, float *dpix, hipStream_t st
 
 void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, hipStream_t st) {
   const ConeArgs &A = P.args;
-  // 32 x 32 column tiles of 32 * 32 / kBpWaves workgroups (kernel mapping)
-  const int blocks = ((A.nx + 31) / 32) * ((A.ny + 31) / 32) * (32 * 32 / kBpWaves);
+  // kBpTile x kBpTile column tiles of kBpTile^2 / kBpWaves workgroups
+  // (kernel mapping)
+  const int blocks = ((A.nx + kBpTile - 1) / kBpTile) * ((A.ny + kBpTile - 1) / kBpTile) *
+                     (kBpTile * kBpTile / kBpWaves);
   const int narr = A.bp_phased ? 1 : A.bp_stride;  // LDS arrays per wavefront
   const size_t lds = bp_lds_bytes(A.nz, narr);
   const BpCol *bc = P.bpcol.as<BpCol>();
