@@ -27,7 +27,7 @@ EXPORTS = (
     "ccpi_cgls", "ccpi_sirt", "ccpi_mlem", "ccpi_regularize", "ccpi_cgls_reg",
     "ccpi_xtek_find_centre",
     "ccpi_pb_cgls_batched", "ccpi_cgls_create", "ccpi_cgls_iterate", "ccpi_cgls_norms", "ccpi_cgls_timings",
-    "ccpi_cgls_destroy", "ccpi_projector_nnz",
+    "ccpi_cgls_destroy", "ccpi_projector_nnz", "ccpi_cone_bp_layout",
     "ccpi_error_string", "ccpi_last_error", "ccpi_version",
     "ccpi_plan_cache_clear", "ccpi_plan_cache_bytes",
 )
@@ -92,12 +92,13 @@ def lib():
     L.ccpi_cgls_timings.argtypes = [vp, dp, ip, dp, ip]
     L.ccpi_cgls_destroy.argtypes = [vp]
     L.ccpi_projector_nnz.argtypes = [gp, C.POINTER(C.c_ulonglong), vp]
+    L.ccpi_cone_bp_layout.argtypes = [gp, ip, ip, ip, ip]
     for name in ("ccpi_pb_f2D", "ccpi_pb_b2D", "ccpi_cb_f2D", "ccpi_cb_b2D",
                  "ccpi_forward_project", "ccpi_backward_project", "ccpi_cgls",
                  "ccpi_sirt", "ccpi_mlem", "ccpi_regularize", "ccpi_cgls_reg",
                  "ccpi_xtek_find_centre", "ccpi_pb_cgls_batched", "ccpi_cgls_create",
                  "ccpi_cgls_iterate", "ccpi_cgls_norms", "ccpi_cgls_timings",
-                 "ccpi_cgls_destroy", "ccpi_projector_nnz"):
+                 "ccpi_cgls_destroy", "ccpi_projector_nnz", "ccpi_cone_bp_layout"):
         getattr(L, name).restype = C.c_int
     L.ccpi_error_string.argtypes = [i]
     L.ccpi_error_string.restype = C.c_char_p

@@ -295,6 +295,20 @@ int ccpi_pb_cgls_batched(const ccpi_geometry *geoms, int count, float *const *b,
   });
 }
 
+int ccpi_cone_bp_layout(const ccpi_geometry *g, int *row_stride, int *phased, int *phases,
+                        int *rows_bound) {
+  return guarded([&] {
+    CCPI_REQUIRE(row_stride && phased && phases && rows_bound, CCPI_E_ARG, "null output");
+    const ccpi::Geometry G = ccpi::Geometry::from(g);
+    CCPI_REQUIRE(G.kind == CCPI_CONE, CCPI_E_ARG, "BP layout query takes a cone geometry");
+    const ccpi::BpLayout L = ccpi::cone_bp_layout(G);
+    *row_stride = L.stride;
+    *phased = L.phased;
+    *phases = L.phases;
+    *rows_bound = L.want;
+  });
+}
+
 const char *ccpi_error_string(int status) {
   switch (status) {
     case CCPI_OK: return "ok";

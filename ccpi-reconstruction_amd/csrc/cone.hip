@@ -1094,6 +1094,78 @@ static size_t bp_lds_bytes(int nz, int narr) {
   return sizeof(float) * (pad + (size_t)kBpWaves * (size_t)narr * pad);
 }
 
+// BP accumulation layout of a cone geometry (host only, no device work).
+// Row stride: at least the largest number of detector rows per slice over
+// all columns.  The walk parameter of any voxel is at least
+// (|source| - R) / L, R the farthest voxel from the axis and L the longest
+// source -> pixel xy distance.
+static int bp_rows_bound(double rows_per_slice) {
+  return std::max(1, (int)std::ceil(std::min(rows_per_slice * 1.05, 1.0e6)));
+}
+
+BpLayout cone_bp_layout(const Geometry &G) {
+  const int nh = G.nh, nv = G.nv, nx = G.nx, ny = G.ny, nz = G.nz;
+  const double sx = G.source[0], sy = G.source[1], sz = G.source[2];
+  BpLayout L;
+  double R = 0.0;
+  for (int cx = 0; cx < 2; cx++)
+    for (int cy = 0; cy < 2; cy++)
+      R = std::max(R, std::hypot(G.origin[0] + cx * nx * G.vsize[0],
+                                 G.origin[1] + cy * ny * G.vsize[1]));
+  double Lmax = 0.0;
+  for (int h : {0, nh - 1})
+    Lmax = std::max(Lmax, std::hypot(G.det_x - sx, G.h_pix[h] - sy));
+  // (positive: the geometry has the source outside the corner circle)
+  const double amin = (std::hypot(sx, sy) - R) / Lmax;
+  double pitch = 1e300;
+  for (int v = 1; v < nv; v++) pitch = std::min(pitch, G.v_pix[v] - G.v_pix[v - 1]);
+  // a source close to the corner circle gives a tiny amin and a bound
+  // past 64 * 16 rows: the phased form then runs one lane per phase,
+  // which keeps every round's cells distinct whatever the rows per slice
+  const double rows_per_slice = G.vsize[2] / (amin * pitch);
+  const int want = bp_rows_bound(rows_per_slice);
+  L.stride = std::min(16, want);
+  // lanes S rows apart are >= 1.05 slices apart: distinct slices
+  L.unique = want <= 16 ? 1 : 0;
+  // a source inside the volume's z range keeps main rows' slices in range
+  // and uniformly spaced rows make min_v / max_v exact row bounds
+  // (they are computed from the first pixel step, c2D.cpp:995-1000)
+  bool uniform = true;
+  const double st0 = G.v_pix[1] - G.v_pix[0];
+  for (int v = 1; v < nv; v++)
+    uniform = uniform && std::fabs((G.v_pix[v] - G.v_pix[v - 1]) - st0) <= 1e-9 * std::fabs(st0);
+  L.main_safe =
+      (uniform && sz >= G.origin[2] && sz < G.origin[2] + (double)nz * G.vsize[2]) ? 1 : 0;
+  // one LDS array per row slot while they fit two workgroups per CU
+  // beside each other; otherwise one array, updated in lane phases
+  L.phased =
+      (L.unique && L.stride <= kBpMaxStride && bp_lds_bytes(nz, L.stride) <= 64 * 1024)
+          ? 0
+          : 1;
+  L.phases = 1;
+  if (L.phased) {
+    // 4, 8 or 16 slots; lanes of one phase are phases * S >= want rows
+    // apart.  Past 4 rows per slice, the S of least (padded rows) x (1 +
+    // 0.11 per extra phase): measured on wide cones, 768 rows: S = 16
+    // 681 ms, S = 8 in 2 phases 756 ms; 384 rows: 83.6 and 47.9 ms.
+    auto phases = [&](int S) { return std::min(64, (want + S - 1) / S); };
+    auto cost = [&](int S) {
+      const double rows = (double)((nv + 64 * S - 1) / (64 * S)) * 64.0 * S;
+      return rows * (1.0 + 0.11 * (phases(S) - 1));
+    };
+    L.stride = want <= 4 ? 4 : (cost(8) <= cost(16) ? 8 : 16);
+    L.phases = phases(L.stride);
+  }
+  CCPI_REQUIRE(bp_lds_bytes(nz, L.phased ? 1 : L.stride) <= 160 * 1024, CCPI_E_LIMIT,
+               "cone BP: nz too large for LDS");
+  // rows of one BP block: 64 lanes bp_stride rows apart (both forms)
+  const int span = 64 * L.stride;
+  L.nvp = (nv + span - 1) / span * span;
+  CCPI_REQUIRE(nv < 65536, CCPI_E_LIMIT, "cone beam: at most 65535 detector rows");
+  L.want = want;
+  return L;
+}
+
 void cone_build_plan(ConePlan &P, hipStream_t st) {
   const Geometry &G = P.g;
   const int na = G.na, nh = G.nh, nv = G.nv, nx = G.nx, ny = G.ny, nz = G.nz;
@@ -1179,62 +1251,14 @@ void cone_build_plan(ConePlan &P, hipStream_t st) {
     hr[2 * a] = std::max(std::min(std::min(h00, h01), std::min(h10, h11)), 0);
     hr[2 * a + 1] = std::min(std::max(std::max(h00, h01), std::max(h10, h11)), nh - 1);
   }
-  // BP row stride: at least the largest number of detector rows per slice
-  // over all columns.  The walk parameter of any voxel is at least
-  // (|source| - R) / L, R the farthest voxel from the axis and L the longest
-  // source -> pixel xy distance.
   {
-    double R = 0.0;
-    for (int cx = 0; cx < 2; cx++)
-      for (int cy = 0; cy < 2; cy++)
-        R = std::max(R, std::hypot(G.origin[0] + cx * nx * G.vsize[0],
-                                   G.origin[1] + cy * ny * G.vsize[1]));
-    double Lmax = 0.0;
-    for (int h : {0, nh - 1})
-      Lmax = std::max(Lmax, std::hypot(G.det_x - sx, G.h_pix[h] - sy));
-    const double amin = std::max((std::hypot(sx, sy) - R) / Lmax, 1e-3);
-    double pitch = 1e300;
-    for (int v = 1; v < nv; v++) pitch = std::min(pitch, G.v_pix[v] - G.v_pix[v - 1]);
-    const double rows_per_slice = G.vsize[2] / (amin * pitch);
-    const int want = std::max(1, (int)std::ceil(std::min(rows_per_slice * 1.05, 1.0e6)));
-    A.bp_stride = std::min(16, want);
-    // lanes S rows apart are >= 1.05 slices apart: distinct slices
-    A.bp_unique = want <= 16 ? 1 : 0;
-    // a source inside the volume's z range keeps main rows' slices in range
-    // and uniformly spaced rows make min_v / max_v exact row bounds
-    // (they are computed from the first pixel step, c2D.cpp:995-1000)
-    bool uniform = true;
-    const double st0 = G.v_pix[1] - G.v_pix[0];
-    for (int v = 1; v < nv; v++)
-      uniform = uniform && std::fabs((G.v_pix[v] - G.v_pix[v - 1]) - st0) <= 1e-9 * std::fabs(st0);
-    A.bp_main_safe =
-        (uniform && sz >= G.origin[2] && sz < G.origin[2] + (double)nz * G.vsize[2]) ? 1 : 0;
-    // one LDS array per row slot while they fit two workgroups per CU
-    // beside each other; otherwise one array, updated in lane phases
-    A.bp_phased =
-        (A.bp_unique && A.bp_stride <= kBpMaxStride && bp_lds_bytes(nz, A.bp_stride) <= 64 * 1024)
-            ? 0
-            : 1;
-    A.bp_phases = 1;
-    if (A.bp_phased) {
-      // 4, 8 or 16 slots; lanes of one phase are phases * S >= want rows
-      // apart.  Past 4 rows per slice, the S of least (padded rows) x (1 +
-      // 0.11 per extra phase): measured on wide cones, 768 rows: S = 16
-      // 681 ms, S = 8 in 2 phases 756 ms; 384 rows: 83.6 and 47.9 ms.
-      auto phases = [&](int S) { return std::min(64, (want + S - 1) / S); };
-      auto cost = [&](int S) {
-        const double rows = (double)((nv + 64 * S - 1) / (64 * S)) * 64.0 * S;
-        return rows * (1.0 + 0.11 * (phases(S) - 1));
-      };
-      A.bp_stride = want <= 4 ? 4 : (cost(8) <= cost(16) ? 8 : 16);
-      A.bp_phases = phases(A.bp_stride);
-    }
-    CCPI_REQUIRE(bp_lds_bytes(nz, A.bp_phased ? 1 : A.bp_stride) <= 160 * 1024, CCPI_E_LIMIT,
-                 "cone BP: nz too large for LDS");
-    // rows of one BP block: 64 lanes bp_stride rows apart (both forms)
-    const int span = 64 * A.bp_stride;
-    A.nvp = (nv + span - 1) / span * span;
-    CCPI_REQUIRE(nv < 65536, CCPI_E_LIMIT, "cone beam: at most 65535 detector rows");
+    const BpLayout L = cone_bp_layout(G);
+    A.bp_stride = L.stride;
+    A.bp_unique = L.unique;
+    A.bp_main_safe = L.main_safe;
+    A.bp_phased = L.phased;
+    A.bp_phases = L.phases;
+    A.nvp = L.nvp;
   }
   // Walk parameter past which a row's ray is outside the volume's z range
   // for good: a row below the source is kLimSlices slices under the bottom

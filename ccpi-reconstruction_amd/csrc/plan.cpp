@@ -5,7 +5,9 @@
 #include <cmath>
 #include <list>
 #include <map>
+#include <condition_variable>
 #include <mutex>
+#include <set>
 
 namespace ccpi {
 
@@ -132,8 +134,14 @@ struct Entry {
   std::shared_ptr<ParPlan> par;
   size_t bytes() const { return cone ? cone->bytes() : par->bytes(); }
 };
+// The mutex guards the list and the set of keys being built; plans are
+// built outside it, so lookups of cached geometries never wait for a build
+// (a 512^3 cone plan takes seconds).  A second caller of a geometry that is
+// being built waits for that build instead of starting its own.
 struct Cache {
   std::mutex mu;
+  std::condition_variable built;
+  std::set<std::string> building;
   std::list<std::pair<std::string, Entry>> lru;  // front = most recent
 };
 Cache &cache() {
@@ -160,40 +168,79 @@ Entry *lookup(Cache &c, const std::string &k) {
   return nullptr;
 }
 
-void insert(Cache &c, const std::string &k, Entry e) {
-  c.lru.emplace_front(k, std::move(e));
-  while (c.lru.size() > kMaxPlans) c.lru.pop_back();
-  auto total = [&c] {
-    size_t b = 0;
-    for (auto &q : c.lru) b += q.second.bytes();
-    return b;
-  };
-  while (c.lru.size() > 1 && total() > kMaxPlanBytes) c.lru.pop_back();
+size_t total_bytes(const Cache &c) {
+  size_t b = 0;
+  for (auto &q : c.lru) b += q.second.bytes();
+  return b;
+}
+
+// Device bytes a plan of g will hold, from the table sizes: the cone walk
+// lists about one entry per crossed xy cell of each ray and the BP lists
+// about two detector columns per angle and voxel column.  Used to make room
+// before a build, so the cache's peak is not the old plans plus the new one.
+size_t plan_estimate(const Geometry &g) {
+  if (g.kind != CCPI_CONE) return (size_t)g.na * 48 + (size_t)(g.nh + g.nv + 2 * g.nz) * 8;
+  const size_t rays = (size_t)g.na * g.nh, cols = (size_t)g.nx * g.ny;
+  return rays * (48 + 12 * (size_t)(g.nx + g.ny)) + cols * (32 + 32 * (size_t)g.na);
+}
+
+// Drops least recently used plans until `incoming` more bytes fit; callers
+// still holding a dropped plan keep it alive until they release it.
+void make_room(Cache &c, size_t incoming) {
+  while (!c.lru.empty() &&
+         (c.lru.size() + 1 > kMaxPlans || total_bytes(c) + incoming > kMaxPlanBytes))
+    c.lru.pop_back();
+}
+
+template <class Build>
+Entry find_or_build(const Geometry &g, Build build) {
+  Cache &c = cache();
+  const std::string k = dev_key(g);
+  std::unique_lock<std::mutex> lk(c.mu);
+  for (;;) {
+    if (Entry *e = lookup(c, k)) return *e;
+    if (!c.building.count(k)) break;
+    c.built.wait(lk);
+  }
+  c.building.insert(k);
+  make_room(c, plan_estimate(g));
+  lk.unlock();
+  Entry e;
+  try {
+    e = build();
+  } catch (...) {
+    lk.lock();
+    c.building.erase(k);
+    c.built.notify_all();
+    throw;
+  }
+  lk.lock();
+  c.building.erase(k);
+  make_room(c, e.bytes());
+  c.lru.emplace_front(k, e);
+  c.built.notify_all();
+  return e;
 }
 }  // namespace
 
 std::shared_ptr<ConePlan> cone_plan(const Geometry &g, hipStream_t st) {
-  Cache &c = cache();
-  const std::string k = dev_key(g);
-  std::lock_guard<std::mutex> lock(c.mu);
-  if (Entry *e = lookup(c, k)) return e->cone;
-  auto P = std::make_shared<ConePlan>();
-  P->g = g;
-  cone_build_plan(*P, st);
-  insert(c, k, Entry{P, nullptr});
-  return P;
+  return find_or_build(g, [&] {
+           auto P = std::make_shared<ConePlan>();
+           P->g = g;
+           cone_build_plan(*P, st);
+           return Entry{P, nullptr};
+         })
+      .cone;
 }
 
 std::shared_ptr<ParPlan> par_plan(const Geometry &g, hipStream_t st) {
-  Cache &c = cache();
-  const std::string k = dev_key(g);
-  std::lock_guard<std::mutex> lock(c.mu);
-  if (Entry *e = lookup(c, k)) return e->par;
-  auto P = std::make_shared<ParPlan>();
-  P->g = g;
-  par_build_plan(*P, st);
-  insert(c, k, Entry{nullptr, P});
-  return P;
+  return find_or_build(g, [&] {
+           auto P = std::make_shared<ParPlan>();
+           P->g = g;
+           par_build_plan(*P, st);
+           return Entry{nullptr, P};
+         })
+      .par;
 }
 
 void plan_cache_clear() {
@@ -205,9 +252,7 @@ void plan_cache_clear() {
 long long plan_cache_bytes() {
   Cache &c = cache();
   std::lock_guard<std::mutex> lock(c.mu);
-  long long b = 0;
-  for (auto &e : c.lru) b += (long long)e.second.bytes();
-  return b;
+  return (long long)total_bytes(c);
 }
 
 // ---- projector pair -------------------------------------------------------
@@ -226,15 +271,36 @@ void Projector::fp(const float *vol, float *sino, hipStream_t st) const {
     par_fp(*par, vol, sino, st);
 }
 
-void Projector::bp(const float *sino, float *vol, hipStream_t st) const {
-  if (cone) {
-    ConePlan &P = *cone;
-    P.dpix.alloc(cone_dpix_floats(P) * sizeof(float));
-    cone_scale(P, sino, P.dpix.as<float>(), st);
-    cone_bp_scaled(P, P.dpix.as<float>(), vol, st);
-  } else {
+size_t Projector::bp_scratch_floats() const { return cone ? cone_dpix_floats(*cone) : 0; }
+
+void Projector::bp_scaled(const float *dpix, float *vol, hipStream_t st) const {
+  CCPI_REQUIRE(cone, CCPI_E_ARG, "pre-scaled backprojection is a cone-beam form");
+  cone_bp_scaled(*cone, dpix, vol, st);
+}
+
+void Projector::bp(const float *sino, float *vol, hipStream_t st, float *scratch) const {
+  if (!cone) {
     par_bp(*par, sino, vol, st);
+    return;
   }
+  const ConePlan &P = *cone;
+  if (scratch) {
+    cone_scale(P, sino, scratch, st);
+    cone_bp_scaled(P, scratch, vol, st);
+    return;
+  }
+  SharedScratch &S = const_cast<SharedScratch &>(P.scratch);
+  std::lock_guard<std::mutex> lk(S.mu);
+  if (!S.done) CCPI_HIP(hipEventCreateWithFlags(&S.done, hipEventDisableTiming));
+  if (!S.buf.p) {
+    S.buf.alloc(cone_dpix_floats(P) * sizeof(float));
+  } else {
+    // the previous caller's BP (any stream) has finished reading the copy
+    CCPI_HIP(hipStreamWaitEvent(st, S.done, 0));
+  }
+  cone_scale(P, sino, S.buf.as<float>(), st);
+  cone_bp_scaled(P, S.buf.as<float>(), vol, st);
+  CCPI_HIP(hipEventRecord(S.done, st));
 }
 
 }  // namespace ccpi

@@ -11,6 +11,7 @@
 
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -74,18 +75,40 @@ struct DeviceBuffer {
   T *as() const { return static_cast<T *>(p); }
 };
 
+// Scaled-sinogram scratch (y * d_conv, the reference's limited_memory =
+// false copy, c2D.cpp:1105-1114) of the cone BP calls that bring none of
+// their own.  One per plan, shared by every such caller of the geometry:
+// a caller holds `mu` while it enqueues the scaling and the BP, and waits
+// on the device for `done` (recorded after the previous caller's BP), so
+// calls from different streams or host threads run one after another on
+// the device and never overwrite a copy another BP still reads.
+struct SharedScratch {
+  mutable std::mutex mu;
+  DeviceBuffer buf;
+  hipEvent_t done = nullptr;
+  SharedScratch() = default;
+  SharedScratch(const SharedScratch &) = delete;
+  SharedScratch &operator=(const SharedScratch &) = delete;
+  ~SharedScratch() {
+    if (done) (void)hipEventDestroy(done);
+  }
+  size_t bytes() const {
+    std::lock_guard<std::mutex> lk(mu);
+    return buf.bytes;
+  }
+};
+
 struct ConePlan {
   Geometry g;
   ConeArgs args{};
   DeviceBuffer atab, hpix, hrange, dz, iz, vz, dconv, offs, rays, ents, eai, alim;
   DeviceBuffer bpcol, bprec;  // backprojection lists
-  DeviceBuffer dpix;    // scaled sinogram scratch of the BP
+  SharedScratch scratch;      // BP scratch of callers without a workspace
   uint64_t n_entries = 0;
   size_t bytes() const {
     return atab.bytes + hpix.bytes + hrange.bytes + dz.bytes + iz.bytes +
            vz.bytes + dconv.bytes + offs.bytes + rays.bytes + ents.bytes + eai.bytes + alim.bytes +
-           bpcol.bytes + bprec.bytes +
-           dpix.bytes;
+           bpcol.bytes + bprec.bytes + scratch.bytes();
   }
 };
 
@@ -102,6 +125,14 @@ std::shared_ptr<ConePlan> cone_plan(const Geometry &g, hipStream_t st);
 std::shared_ptr<ParPlan> par_plan(const Geometry &g, hipStream_t st);
 void plan_cache_clear();
 long long plan_cache_bytes();
+
+// Accumulation layout of the cone BP for a geometry (host only): row stride
+// S, whether it is the phased one-array form and its lane phases, the bound
+// on detector rows per slice it was chosen for, and the padded row length.
+struct BpLayout {
+  int stride = 1, unique = 1, main_safe = 0, phased = 0, phases = 1, nvp = 0, want = 1;
+};
+BpLayout cone_bp_layout(const Geometry &g);
 
 // launchers (cone.hip, parallel.hip)
 void cone_build_plan(ConePlan &P, hipStream_t st);
@@ -125,7 +156,14 @@ struct Projector {
   std::shared_ptr<ParPlan> par;
   explicit Projector(const Geometry &g, hipStream_t st);
   void fp(const float *vol, float *sino, hipStream_t st) const;
-  void bp(const float *sino, float *vol, hipStream_t st) const;
+  // scratch: bp_scratch_floats() floats owned by the caller and used by
+  // no other call meanwhile (a solver's workspace); nullptr takes the
+  // plan's shared scratch
+  void bp(const float *sino, float *vol, hipStream_t st, float *scratch = nullptr) const;
+  // BP of an already scaled sinogram in the padded BP layout (cone only):
+  // a solver's fused update kernels write it
+  void bp_scaled(const float *dpix, float *vol, hipStream_t st) const;
+  size_t bp_scratch_floats() const;
   const Geometry &geometry() const { return cone ? cone->g : par->g; }
 };
 

@@ -262,7 +262,9 @@ void dev_fill(float *p, size_t n, float v, hipStream_t st) {
 
 Workspace::~Workspace() = default;
 
-void Workspace::ensure(size_t nvox, size_t npix) {
+void Workspace::ensure(const Projector &P) {
+  const size_t nvox = P.geometry().n_vox(), npix = P.geometry().n_pix();
+  if (const size_t nd = P.bp_scratch_floats()) dscale.alloc(nd * 4);
   v1.alloc(nvox * 4);
   v2.alloc(nvox * 4);
   v3.alloc(nvox * 4);
@@ -280,12 +282,12 @@ static void sumsq(const float *x, size_t n, void *part, hipStream_t st) {
 }
 
 void CglsRun::start(hipStream_t st) {
-  const size_t nvox = P.geometry().n_vox(), npix = P.geometry().n_pix();
-  W.ensure(nvox, npix);
+  const size_t nvox = P.geometry().n_vox();
+  W.ensure(P);
   norms_dev.alloc((size_t)std::max(iters, 1) * sizeof(double));
   float *d = W.v1.as<float>();
   dev_fill(d, nvox, 0.f, st);
-  P.bp(b, d, st);
+  P.bp(b, d, st, W.bp_scratch());
   if (fp32) {
     sumsq<float>(d, nvox, W.part.p, st);
     hipLaunchKernelGGL(cgls_init_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
@@ -342,7 +344,7 @@ void CglsRun::iterate(int it, hipStream_t st) {
                      0, st, x, d, nvox, b, Ad, npix, sc);
   dev_fill(s, nvox, 0.f, st);
   if (log) log->mark(log->bp, st);
-  P.bp(b, s, st);
+  P.bp(b, s, st, W.bp_scratch());
   if (log) log->mark(log->bp, st);
   if (fp32) {
     sumsq<float>(s, nvox, W.part.p, st);
@@ -385,7 +387,7 @@ void run_cgls_reg(const Projector &P, float *b, float *x, int iters, int kind, d
   const Geometry &G = P.geometry();
   check_reg_dims(G.nx, G.ny, G.nz);
   const size_t nvox = G.n_vox(), npix = G.n_pix();
-  W.ensure(nvox, npix);
+  W.ensure(P);
   W.v4.alloc(nvox * 4);
   W.part2.alloc(kRedBlocks * sizeof(double));
   const size_t nscr = reg_scratch_floats(kind, nvox);
@@ -400,7 +402,7 @@ void run_cgls_reg(const Projector &P, float *b, float *x, int iters, int kind, d
   const int gv = grid_for(nvox);
   dev_fill(d, nvox, 0.f, st);
   dev_fill(adv, nvox, 0.f, st);
-  P.bp(b, d, st);
+  P.bp(b, d, st, W.bp_scratch());
   if (fp32) {
     sumsq<float>(d, nvox, W.part.p, st);
     hipLaunchKernelGGL(cgls_init_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
@@ -432,7 +434,7 @@ void run_cgls_reg(const Projector &P, float *b, float *x, int iters, int kind, d
     reg_term(kind, x, L, scr, G.nx, G.ny, G.nz, st);
     hipLaunchKernelGGL(axpy_kernel, dim3(gv), dim3(256), 0, st, x, L, nvox, lam);
     dev_fill(s, nvox, 0.f, st);
-    P.bp(b, s, st);
+    P.bp(b, s, st, W.bp_scratch());
     if (fp32) {
       sumsq<float>(s, nvox, W.part.p, st);
       hipLaunchKernelGGL(cgls_beta_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
@@ -455,7 +457,7 @@ void run_cgls_reg(const Projector &P, float *b, float *x, int iters, int kind, d
 void run_sirt(const Projector &P, const float *b, float *x, int iters, Workspace &W,
               hipStream_t st) {
   const size_t nvox = P.geometry().n_vox(), npix = P.geometry().n_pix();
-  W.ensure(nvox, npix);
+  W.ensure(P);
   float *C = W.v1.as<float>(), *t = W.v2.as<float>(), *ones = W.v3.as<float>();
   float *R = W.p1.as<float>(), *r = W.p2.as<float>();
   // R = 1 / (A 1), C = 1 / (A^T 1) (sirt.cpp:31-48)
@@ -465,7 +467,7 @@ void run_sirt(const Projector &P, const float *b, float *x, int iters, Workspace
   hipLaunchKernelGGL(invert_min_kernel, dim3(grid_for(npix)), dim3(256), 0, st, R, npix);
   dev_fill(r, npix, 1.f, st);
   dev_fill(C, nvox, 0.f, st);
-  P.bp(r, C, st);
+  P.bp(r, C, st, W.bp_scratch());
   hipLaunchKernelGGL(invert_min_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, C, nvox);
   for (int it = 0; it < iters; it++) {
     dev_fill(r, npix, 0.f, st);
@@ -473,7 +475,7 @@ void run_sirt(const Projector &P, const float *b, float *x, int iters, Workspace
     hipLaunchKernelGGL(sirt_residual_kernel, dim3(grid_for(npix)), dim3(256), 0, st, r, b, R,
                        npix);
     dev_fill(t, nvox, 0.f, st);
-    P.bp(r, t, st);
+    P.bp(r, t, st, W.bp_scratch());
     hipLaunchKernelGGL(sirt_update_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, x, t, C,
                        nvox);
   }
@@ -483,21 +485,21 @@ void run_sirt(const Projector &P, const float *b, float *x, int iters, Workspace
 void run_mlem(const Projector &P, const float *b, float *x, int iters, Workspace &W,
               hipStream_t st) {
   const size_t nvox = P.geometry().n_vox(), npix = P.geometry().n_pix();
-  W.ensure(nvox, npix);
+  W.ensure(P);
   float *N = W.v1.as<float>(), *t = W.v2.as<float>();
   float *r = W.p1.as<float>();
   // x = 1; N = 1 / (A^T 1) keeping zeros (mlem.cpp:30-41)
   dev_fill(x, nvox, 1.f, st);
   dev_fill(r, npix, 1.f, st);
   dev_fill(N, nvox, 0.f, st);
-  P.bp(r, N, st);
+  P.bp(r, N, st, W.bp_scratch());
   hipLaunchKernelGGL(invert_nz_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, N, nvox);
   for (int it = 0; it < iters; it++) {
     dev_fill(r, npix, 0.f, st);
     P.fp(x, r, st);
     hipLaunchKernelGGL(mlem_ratio_kernel, dim3(grid_for(npix)), dim3(256), 0, st, r, b, npix);
     dev_fill(t, nvox, 0.f, st);
-    P.bp(r, t, st);
+    P.bp(r, t, st, W.bp_scratch());
     hipLaunchKernelGGL(mlem_update_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, x, N, t,
                        nvox);
   }

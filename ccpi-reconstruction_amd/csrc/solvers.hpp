@@ -24,8 +24,11 @@ struct Workspace {
   DeviceBuffer scal;         // CglsScalars
   DeviceBuffer v4, part2;    // regularised CGLS: Ad_vox, its partials
   DeviceBuffer tvd;          // TV term: D1, D2, D3
+  DeviceBuffer dscale;       // cone BP input scaled by d_conv (padded rows)
   ~Workspace();
-  void ensure(size_t nvox, size_t npix);
+  void ensure(const Projector &P);
+  // the workspace's BP scratch (nullptr for the parallel beam)
+  float *bp_scratch() const { return dscale.p ? dscale.as<float>() : nullptr; }
 };
 
 // regularize.hip: Tikhonov / TV terms (tikhonov.cpp, total_v.cpp)

@@ -182,6 +182,17 @@ int ccpi_pb_cgls_batched(const ccpi_geometry *geoms, int count,
                          float *const *b, float *const *x, int iterations,
                          double *norms, int flags, void *stream);
 
+/* BP accumulation layout the library picks for a cone geometry, computed
+   on the host only (no device work, usable without a GPU): *row_stride =
+   detector rows per lane (S), *phased = 1 for the one-array form whose
+   lanes update in *phases rounds, *rows_bound = the bound on detector rows
+   per voxel slice over all columns that the choice guarantees (lanes of one
+   round are >= that many rows apart, or alone in their round).  No
+   counterpart in the reference: its b2D accumulates each voxel column on
+   one host thread (c2D.cpp:1016-1046). */
+int ccpi_cone_bp_layout(const ccpi_geometry *g, int *row_stride, int *phased,
+                        int *phases, int *rows_bound);
+
 /* ---- utilities --------------------------------------------------------- */
 const char *ccpi_error_string(int status);
 /* Detail of the last failing call on this host thread (empty after a

@@ -13,86 +13,24 @@ default test cases), 3, 4, and the phased form with:
     volume's z range); two blocks, the second padded, with rows outside it;
   * S = 8 in two phases (300 rows), S = 16 in one and in two phases (wide
     cones: the volume's corners close to the source);
-  * S = 4 (arrays beyond the LDS budget).
+  * S = 4 (arrays beyond the LDS budget);
+  * S = 8 in 64 phases: the source 0.002 mm outside the volume's corner
+    circle, where thousands of rows fall on one slice near the corners.
 Each is checked for FP and BP parity (1e-6 / 1e-5 of the maximum, as in
 test_gpu_parity) and the adjoint identity.  ``_path`` restates the host's
 choice so that a geometry that no longer reaches its intended kernel fails
 here instead of silently testing another one.
 """
 
-import math
-
 import numpy as np
 import pytest
 
 torch = pytest.importorskip("torch")
 
-import oracle
-from geometries import CB, centred, ellipsoid_phantom, oracle_geom
+from bp_cases import CASES, NOT_ADJOINT, bp_path as _path
+from geometries import ellipsoid_phantom, oracle_geom
 
 pytestmark = pytest.mark.gpu
-
-A12 = np.linspace(0, 2 * np.pi, 12, endpoint=False) + 0.1
-
-CASES = {
-    "s1": (CB((10, 10, 8), (1, 1, 1), np.linspace(0, 2 * np.pi, 12, endpoint=False) + 0.1,
-              centred(20, 1.6), centred(8, 3.0), 40.0, 80.0), (1, False, 1)),
-    "s3": (CB((10, 10, 6), (1, 1, 1), np.linspace(0, 2 * np.pi, 12, endpoint=False) + 0.1,
-              centred(20, 1.6), centred(20, 0.95), 40.0, 80.0), (3, False, 1)),
-    "s4": (CB((10, 10, 6), (1, 1, 1), np.linspace(0, 2 * np.pi, 12, endpoint=False) + 0.1,
-              centred(20, 1.6), centred(24, 0.75), 40.0, 80.0), (4, False, 1)),
-    "phased_s8_small": (CB((8, 8, 4), (1, 1, 1), np.linspace(0, 2 * np.pi, 10, endpoint=False),
-                           centred(16, 1.6), centred(40, 0.4), 40.0, 80.0), (8, True, 1)),
-    "phased_s8_offz": (CB((8, 8, 4), (1, 1, 1), np.linspace(0, 2 * np.pi, 10, endpoint=False),
-                          centred(16, 1.6), centred(40, 0.4), 40.0, 80.0, source_z=1.7),
-                       (8, True, 1)),
-    "phased_s8_main": (CB((12, 12, 200), (1, 1, 1), A12, centred(20, 1.6), centred(460, 0.5),
-                          20.0, 40.0), (8, True, 1)),
-    "phased_s8_blocks": (CB((12, 12, 160), (1, 1, 0.25), A12, centred(20, 1.6),
-                            centred(700, 0.17), 20.0, 40.0), (8, True, 1)),
-    "phased_s16": (CB((12, 12, 40), (1, 1, 1), A12, centred(20, 1.6), centred(700, 0.25),
-                      24.0, 48.0), (16, True, 1)),
-    "phased_s8_2ph": (CB((12, 12, 40), (1, 1, 1), A12, centred(20, 1.6), centred(300, 0.25),
-                         24.0, 48.0), (8, True, 2)),
-    "phased_s16_2ph": (CB((12, 12, 40), (1, 1, 1), A12, centred(20, 1.6), centred(700, 0.2),
-                          24.0, 48.0), (16, True, 2)),
-    "phased_s4_lds": (CB((4, 4, 1500), (1, 1, 0.05), np.linspace(0, 2 * np.pi, 8, endpoint=False),
-                         centred(10, 1.2), centred(3000, 0.05), 40.0, 80.0), (4, True, 1)),
-}
-
-
-# With the source inside the volume's top slice the reference's b2D is not
-# the transpose of its f2D (the oracle's own <Ax, y> and <x, A^T y> differ
-# by 31 % for this case; test_oracle_pins maps the heights): parity only.
-NOT_ADJOINT = {"phased_s8_offz"}
-
-
-def _path(g):
-    """(row stride, phased, phases) as the host chooses them (cone.hip plan
-    build)."""
-    nx, ny, nz = g.shape
-    vs, o = g.vsize, g.origin
-    R = max(math.hypot(o[0] + cx * nx * vs[0], o[1] + cy * ny * vs[1])
-            for cx in (0, 1) for cy in (0, 1))
-    Lmax = max(math.hypot(g.det_x - g.source[0], g.h_pix[i] - g.source[1]) for i in (0, -1))
-    amin = max((math.hypot(g.source[0], g.source[1]) - R) / Lmax, 1e-3)
-    want = max(1, math.ceil(vs[2] / (amin * np.diff(g.v_pix).min()) * 1.05))
-    s = min(16, want)
-    pad = (nz + 2 + 3) & ~3
-    lds = 4 * (pad + 4 * s * pad)
-    phased = not (want <= 16 and s <= 4 and lds <= 64 * 1024)
-    if not phased:
-        return s, False, 1
-    nv = len(g.v_pix)
-
-    def phases(s):
-        return min(64, -(-want // s))
-
-    def cost(s):
-        return -(-nv // (64 * s)) * 64 * s * (1.0 + 0.11 * (phases(s) - 1))
-
-    s = 4 if want <= 4 else (8 if cost(8) <= cost(16) else 16)
-    return s, True, phases(s)
 
 
 def _geom(g):
