@@ -4,8 +4,9 @@
 Workload: 512^3 float32 volume of 0.5 mm voxels (256 mm cube centred on the
 rotation axis), 720 projections over 360 deg, flat 768 x 768 detector of
 1.0 mm pixels, source-object 500 mm, source-detector 1000 mm; seeded
-30-ellipsoid phantom (densities U[0, 0.05] /mm, seed 2), Poisson noise at
-I0 = 1e4 on the log-transformed sinogram.
+30-ellipsoid phantom (densities U[0, 0.05] /mm, seed 2) built on the host,
+its projection on the device, Poisson noise at I0 = 1e4 drawn on the host
+(numpy PCG64, seed 2) on the log-transformed sinogram.
 
 A step is one CGLS iteration (one forward projection, one backprojection,
 FP64 alpha/beta reductions and the fused vector updates) with every input
@@ -14,16 +15,25 @@ already resident in HBM.  `value` is CGLS iterations per second.
 roofline: the dominant projector kernel's algorithmic bytes per launch,
 4 * nnz(A) + 4 * (N_rays + N_vox) with nnz counted exactly on the device,
 divided by its mean launch time from HIP events recorded on the launch
-stream inside the timed iterations, against 8 TB/s.
+stream inside the timed iterations, against 8 TB/s.  `traffic` is the HBM-
+side bytes per launch from the committed counter summary, reported only
+while the summary's source stamp matches the kernel sources.
 
-cpu_baseline: the CPU oracle (a C restatement of the reference's f2D / b2D,
-built with the reference's flags) timed on this host on a sample of the
-projection angles, scaled to one CGLS iteration.
+accuracy: 10 CGLS iterations on the same data as the committed FP64 / float
+oracle runs (tests/golden/accuracy_cone512.npz, identical sinogram checked
+by SHA-256): RMSE against the phantom of the device, of the FP64 oracle and
+of the reference's float arithmetic.
+
+cpu_baseline: the CPU oracle (a C restatement of the reference's f2D / b2D
+and CGLS body, built with the reference's flags) runs one full CGLS
+iteration of this workload on the host (forward and back projection of all
+720 angles and the vector updates, timed inside the oracle).
 """
 
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -35,59 +45,150 @@ for p in (os.path.join(ROOT, "ccpi-reconstruction_amd"), os.path.join(ROOT, "ora
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+KERNEL_SOURCES = ("cone.hip", "parallel.hip", "par_body.hpp", "common.hpp")
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def source_stamp() -> str:
+    """SHA-256 (first 16 hex digits) of the projector kernel sources."""
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "ccpi-reconstruction_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(kernel: str):
     """HBM-side bytes per launch of `kernel` from the committed counter
-    summary (FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy); the
-    summary holds one `TRAFFIC <kernel> <bytes>` line per kernel."""
+    summary (FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy): one
+    `TRAFFIC <kernel> <bytes>` line per kernel and a `SOURCES <stamp>` line.
+    Returns (bytes or None, "current" / "stale" / "missing")."""
     path = os.path.join(ROOT, "profiles", "cone512_projectors_counters.md")
+    val, stamp = None, None
     try:
         with open(path) as f:
             for line in f:
                 parts = line.split()
                 if len(parts) == 3 and parts[0] == "TRAFFIC" and parts[1] == kernel:
-                    return int(parts[2])
+                    val = int(parts[2])
+                if len(parts) == 2 and parts[0] == "SOURCES":
+                    stamp = parts[1]
+    except OSError:
+        return None, "missing"
+    if val is None:
+        return None, "missing"
+    if stamp != source_stamp():
+        return None, "stale"
+    return val, "current"
+
+
+def sha256(a) -> str:
+    import numpy as np
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return None
+    return "unknown"
 
 
-def cpu_baseline(geom, phantom_np, n_sample: int):
-    """Time the oracle's cone f2D + b2D on `n_sample` evenly spaced angles
-    of the workload, scaled to all angles = one CGLS iteration."""
+def headline_data(geom, dev):
+    """Phantom (host), its projection (device), Poisson noise (host): the
+    data of the committed oracle runs.  Returns the phantom (host, device),
+    the noisy sinogram (host, device) and the two sinograms' SHA-256."""
+    import torch
+
+    from ccpi.reconstruction import workloads as W
+    from ccpi.reconstruction.projectors import forward_project
+    ph = W.ellipsoid_phantom_host(geom, 30, seed=2, dens=(0.0, 0.05))
+    ph_dev = torch.from_numpy(ph).to(dev)
+    clean = forward_project(geom, ph_dev).cpu().numpy()   # also builds the plan
+    sha_clean = sha256(clean)
+    noisy = W.poisson_log_host(clean, 1e4, seed=2)
+    del clean
+    return ph, ph_dev, noisy, torch.from_numpy(noisy).to(dev), sha_clean, sha256(noisy)
+
+
+def accuracy_512(geom, b, phantom, sha_clean, sha_noisy, iters=10):
+    """CGLS from x = 0 on the headline data against the committed oracle
+    runs of the same data."""
     import numpy as np
+    import torch
 
+    from ccpi.reconstruction.projectors import CglsSession
+    path = os.path.join(ROOT, "tests", "golden", "accuracy_cone512.npz")
+    out = {"iterations": iters}
+    x = torch.zeros(geom.shape, dtype=torch.float32, device=b.device)
+    s = CglsSession(geom, b.clone(), x, iters)
+    rmse = []
+    for _ in range(iters):
+        s.iterate(1)
+        rmse.append(float(torch.sqrt(torch.mean((x - phantom).double() ** 2))))
+    norms = s.norms()
+    s.close()
+    out["rmse_gpu"] = rmse[-1]
+    out["norm_atr_gpu"] = float(norms[-1])
+    if not os.path.exists(path):
+        out["oracle"] = "fixture missing"
+        return out
+    f = np.load(path)
+    n = min(iters, len(f["rmse_f64"]))
+    same = sha_clean == str(f["sha256_clean"]) and sha_noisy == str(f["sha256_noisy"])
+    out.update({
+        "same_data_as_oracle_runs": same,
+        "rmse_fp64_oracle": float(f["rmse_f64"][n - 1]),
+        "rmse_ref_float": float(f["rmse_f32"][n - 1]),
+        "norm_atr_fp64_oracle": float(f["norms_f64"][n - 1]),
+        "norm_atr_ref_float": float(f["norms_f32"][n - 1]),
+    })
+    out["rmse_rel_err_vs_fp64"] = abs(rmse[n - 1] / out["rmse_fp64_oracle"] - 1)
+    out["ref_float_rmse_rel_err_vs_fp64"] = abs(out["rmse_ref_float"] /
+                                                out["rmse_fp64_oracle"] - 1)
+    sub = int(json.loads(str(f["meta"]))["sub"])
+    xs = x[::sub, ::sub, ::sub].cpu().numpy().astype(np.float64)
+    x64 = f["x_f64_sub"].astype(np.float64)
+    nrm = float(np.linalg.norm(x64))
+    out["x_rel_dist_vs_fp64_subsampled"] = float(np.linalg.norm(xs - x64)) / nrm
+    out["ref_float_x_rel_dist_vs_fp64_subsampled"] = float(
+        np.linalg.norm(f["x_f32_sub"].astype(np.float64) - x64)) / nrm
+    return out
+
+
+def cpu_baseline(geom, noisy_np):
+    """One full CGLS iteration of the workload by the oracle on the host
+    (all angles; forward projection, backprojection and vector updates
+    timed inside the oracle; the initial A^T b is not counted)."""
     import oracle
-    sel = np.linspace(0, geom.angles.size, n_sample, endpoint=False).astype(int)
-    og = oracle.Geometry(geom.angles[sel], geom.h_pixels, geom.v_pixels, geom.origin,
+    og = oracle.Geometry(geom.angles, geom.h_pixels, geom.v_pixels, geom.origin,
                          geom.voxel_size, geom.shape, cone=True, source=geom.source,
                          det_x=geom.detector_x)
     t0 = time.perf_counter()
-    y = og.fp(phantom_np)
-    t1 = time.perf_counter()
-    og.bp(y)
-    t2 = time.perf_counter()
-    scale = geom.angles.size / n_sample
-    iter_s = (t1 - t0 + t2 - t1) * scale
+    _, _, _, _, secs = oracle.cgls_track(og, noisy_np, 1)
+    wall = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
-        "value": 1.0 / iter_s,
+        "value": 1.0 / float(secs[0]),
         "unit": "iter/s",
         "cores": cores,
+        "cpu_model": cpu_model(),
         "kind": "port",
-        "sample": (f"oracle cone f2D+b2D (float, OpenMP) on {n_sample} of "
-                   f"{geom.angles.size} angles of the 512^3 workload, "
-                   f"FP {t1 - t0:.2f} s + BP {t2 - t1:.2f} s, scaled x{scale:.0f}"),
+        "sample": (f"one full CGLS iteration of the 512^3 workload (FP + BP of all 720 "
+                   f"angles, float, OpenMP, and the vector updates): {secs[0]:.1f} s; "
+                   f"{wall:.1f} s with the initial A^T b"),
     }
 
 
 def batched_vs_plain(iters: int, small: bool = False):
-    """Config 3: 64 uneven parallel-beam CGLS problems, batched (one
+    """Batched parallel-beam CGLS: 64 uneven problems, batched (one
     persistent launch per step, graph replay) vs the same problems one after
     another through the one-problem path.  Per-iteration time is
     (T(iters) - T(0)) / iters for both, so set-up is excluded alike.
@@ -161,18 +262,20 @@ def batched_vs_plain(iters: int, small: bool = False):
 
 
 def parallel_cfg1(cpu: bool):
-    """Config 1: plain parallel-beam CGLS, 256 x 256 x 16 slices (1 mm),
-    180 angles over [0, 180 deg), 367 x 16 detector, 10-ellipsoid phantom
-    (seed 1, densities U[0.1, 1]), noise-free sinogram, 20 iterations from
-    x0 = 0; the CPU oracle runs the same 20 iterations."""
+    """Plain parallel-beam CGLS, 256 x 256 x 16 slices (1 mm), 180 angles
+    over [0, 180 deg), 367 x 16 detector, 10-ellipsoid phantom (seed 1,
+    densities U[0.1, 1]), noise-free sinogram, 20 iterations from x0 = 0;
+    the CPU oracle runs the same 20 iterations.  The projectors' launch
+    times come from HIP events of a timed CGLS session (roofline)."""
     import numpy as np
     import torch
 
     from ccpi.reconstruction import workloads as W
-    from ccpi.reconstruction.projectors import cgls, forward_project, projector_nnz
+    from ccpi.reconstruction.projectors import (CglsSession, cgls, forward_project,
+                                                projector_nnz)
 
     g = W.parallel_geometry(256, 16, 180, 367)
-    ph = W.ellipsoid_phantom(g, 10, seed=1, dens=(0.1, 1.0))
+    ph = torch.from_numpy(W.ellipsoid_phantom_host(g, 10, seed=1, dens=(0.1, 1.0))).cuda()
     b = forward_project(g, ph)
     cgls(g, b, 1)                                # plan + warm kernels
     torch.cuda.synchronize()
@@ -184,10 +287,24 @@ def parallel_cfg1(cpu: bool):
         reps.append(time.perf_counter() - t0)
     secs = sorted(reps)[1]
     rmse = float(torch.sqrt(torch.mean((x - ph) ** 2)))
+    nnz = projector_nnz(g)
+    bytes_alg = 4 * nnz + 4 * (int(np.prod(g.shape)) + int(np.prod(g.sino_shape)))
+    xs = torch.zeros(g.shape, dtype=torch.float32, device="cuda")
+    s = CglsSession(g, b.clone(), xs, 20, timing=True)
+    s.iterate(20)
+    fp_ms, nf, bp_ms, nb = s.timings()
+    s.close()
+    fp_ms, bp_ms = fp_ms / max(nf, 1), bp_ms / max(nb, 1)
+    dom, dom_ms = ("par_bp_kernel", bp_ms) if bp_ms >= fp_ms else ("par_fp_kernel", fp_ms)
+    ach = bytes_alg / (dom_ms * 1e-3) / 1e9
     out = {"workload": "parallel-beam CGLS 256x256x16, 180 angles, 367x16 detector, "
                        "20 iterations, noise-free",
            "iter_s": 20.0 / secs, "ms_per_iter": secs / 20 * 1e3,
-           "rmse_vs_phantom": rmse, "nnz": projector_nnz(g)}
+           "rmse_vs_phantom": rmse, "nnz": nnz, "fp_ms": fp_ms, "bp_ms": bp_ms,
+           "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "bytes_alg": bytes_alg,
+                        "fp_frac": bytes_alg / (fp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "bp_frac": bytes_alg / (bp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
     if cpu:
         import oracle
         og = oracle.Geometry(g.angles, g.h_pixels, g.v_pixels, g.origin, g.voxel_size,
@@ -230,8 +347,8 @@ def cone_cgls_reg(geom, b, iters: int):
 
 
 def cone_sirt_mlem(geom, b, iters: int):
-    """Config 2: cone-beam SIRT and MLEM at the 512^3 geometry, `iters`
-    iterations each on the headline sinogram; per-iteration time is
+    """Cone-beam SIRT and MLEM at the 512^3 geometry, `iters` iterations
+    each on the headline sinogram; per-iteration time is
     (T(iters) - T(1)) / (iters - 1), so the set-up projections are excluded."""
     import torch
 
@@ -247,6 +364,55 @@ def cone_sirt_mlem(geom, b, iters: int):
             ts.append(time.perf_counter() - t0)
         out[f"{name}_iter_s"] = (iters - 1) / max(ts[1] - ts[0], 1e-9)
     out["workload"] = f"cone-beam SIRT and MLEM 512^3, {iters} iterations each"
+    return out
+
+
+def errors_vs_fp64(iters: int = 10):
+    """Error against the FP64 oracle of every iteration body at the 512^3
+    scan's geometry scaled to 128^3 (1 mm voxels, 180 projections, 192^2
+    detector of 2 mm, SOD 500, SDD 1000), 30-ellipsoid phantom (seed 2),
+    Poisson noise I0 = 1e4: relative l2 distance of the device result and
+    of the oracle's float result (the reference's arithmetic) to the FP64
+    oracle's, after `iters` iterations."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from ccpi.reconstruction import workloads as W
+    from ccpi.reconstruction.projectors import (TIKHONOV, TV, cgls, cgls_reg, forward_project,
+                                                mlem, sirt)
+    g = W.cone_geometry(128, 1.0, 180, 192, 2.0, 500.0, 1000.0)
+    og = oracle.Geometry(g.angles, g.h_pixels, g.v_pixels, g.origin, g.voxel_size, g.shape,
+                         cone=True, source=g.source, det_x=g.detector_x)
+    ph = W.ellipsoid_phantom_host(g, 30, seed=2, dens=(0.0, 0.05))
+    clean = forward_project(g, torch.from_numpy(ph).cuda()).cpu().numpy()
+    bn = W.poisson_log_host(clean, 1e4, seed=2)
+    b = torch.from_numpy(bn).cuda()
+    runs = {
+        "cgls": (lambda: cgls(g, b, iters)[0],
+                 lambda dt: oracle.cgls(og, bn.astype(dt), iters, dtype=dt)[0]),
+        "sirt": (lambda: sirt(g, b, iters),
+                 lambda dt: oracle.sirt(og, bn.astype(dt), iters, dtype=dt)),
+        "mlem": (lambda: mlem(g, b, iters),
+                 lambda dt: oracle.mlem(og, bn.astype(dt), iters, dtype=dt)),
+        "mlem_accum_fp64": (lambda: mlem(g, b, iters, accum_fp64=True),
+                            lambda dt: oracle.mlem(og, bn.astype(dt), iters, dtype=dt)),
+        "cgls_tikhonov": (lambda: cgls_reg(g, b, iters, TIKHONOV, 0.1)[0],
+                          lambda dt: oracle.cgls_reg(og, bn.astype(dt), iters, oracle.TIKHONOV,
+                                                     0.1, dtype=dt)[0]),
+        "cgls_tv": (lambda: cgls_reg(g, b, iters, TV, 1e-4)[0],
+                    lambda dt: oracle.cgls_reg(og, bn.astype(dt), iters, oracle.TV, 1e-4,
+                                               dtype=dt)[0]),
+    }
+    out = {"workload": "cone 128^3 (1 mm), 180 proj / 360 deg, 192^2 detector (2 mm), "
+                       f"SOD 500, SDD 1000, Poisson I0 1e4, {iters} iterations"}
+    for name, (dev_fn, or_fn) in runs.items():
+        xd = dev_fn().cpu().numpy().astype(np.float64)
+        x64 = or_fn(np.float64)
+        x32 = or_fn(np.float32).astype(np.float64)
+        nrm = float(np.linalg.norm(x64))
+        out[name] = {"err_vs_fp64": float(np.linalg.norm(xd - x64)) / nrm,
+                     "ref_float_err_vs_fp64": float(np.linalg.norm(x32 - x64)) / nrm}
     return out
 
 
@@ -304,14 +470,16 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=12,
-                    help="angles in the CPU-oracle sample (0 disables it)")
+    ap.add_argument("--cpu", type=int, default=1,
+                    help="run the CPU oracle legs (baseline iteration, checkers)")
     ap.add_argument("--batched-iters", type=int, default=20,
                     help="CGLS iterations of the batched-vs-plain comparison (0 skips it)")
     ap.add_argument("--extras", type=int, default=1,
-                    help="also measure config 1 (parallel CGLS) and config 2 SIRT/MLEM")
+                    help="also measure the parallel CGLS, SIRT/MLEM and regularised bodies")
     ap.add_argument("--wide", type=int, default=1,
                     help="with --extras: also time the wide-cone BP (0 disables it)")
+    ap.add_argument("--accuracy", type=int, default=1,
+                    help="run the 10-iteration accuracy comparison with the oracle runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device to device (the known-size kernel that "
                          "scales FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -325,8 +493,7 @@ def main() -> int:
     import torch
 
     from ccpi.reconstruction import workloads as W
-    from ccpi.reconstruction.projectors import (CglsSession, forward_project,
-                                                projector_nnz)
+    from ccpi.reconstruction.projectors import CglsSession, projector_nnz
 
     dev = torch.device("cuda:0")
     if args.copy_calibration:
@@ -337,11 +504,8 @@ def main() -> int:
         del src, dst
     cfg = W.CONE_512
     geom = W.cone_geometry(**cfg)
-    log("bench: phantom + sinogram")
-    phantom = W.ellipsoid_phantom(geom, 30, seed=2, dens=(0.0, 0.05), device=dev)
-    clean = forward_project(geom, phantom)            # also builds the plan
-    b = W.poisson_log(clean, 1e4, seed=2)
-    del clean
+    log("bench: phantom (host) + sinogram (device) + Poisson noise (host)")
+    ph_np, phantom, noisy_np, b, sha_clean, sha_noisy = headline_data(geom, dev)
     x = torch.zeros(geom.shape, dtype=torch.float32, device=dev)
     nnz = projector_nnz(geom)
     n_vox = int(np.prod(geom.shape))
@@ -370,6 +534,12 @@ def main() -> int:
     achieved = bytes_alg / (dom_ms * 1e-3) / 1e9
     rmse = float(torch.sqrt(torch.mean((x - phantom) ** 2)))
     log(f"bench: {secs / args.steps * 1e3:.1f} ms/iter, FP {fp_ms:.1f} ms, BP {bp_ms:.1f} ms")
+    del x
+
+    accuracy = None
+    if args.accuracy:
+        log("bench: accuracy against the oracle runs (10 iterations)")
+        accuracy = accuracy_512(geom, b, phantom, sha_clean, sha_noisy)
 
     batched = None
     if args.batched_iters > 0:
@@ -378,10 +548,13 @@ def main() -> int:
 
     extras = None
     if args.extras:
-        log("bench: config 1 (parallel CGLS) and config 2 SIRT / MLEM")
-        extras = {"cfg1": parallel_cfg1(cpu=args.cpu_sample > 0),
+        log("bench: parallel CGLS, cone SIRT / MLEM, regularised CGLS")
+        extras = {"cfg1": parallel_cfg1(cpu=args.cpu > 0),
                   "cfg2": cone_sirt_mlem(geom, b, 4),
                   "cgls_reg": cone_cgls_reg(geom, b, 3)}
+        if args.cpu > 0:
+            log("bench: errors against the FP64 oracle (128^3)")
+            extras["err_vs_fp64"] = errors_vs_fp64()
         if args.batched_iters > 0:
             log("bench: batched vs plain on small problems")
             extras["batched_small"] = batched_vs_plain(args.batched_iters, small=True)
@@ -390,10 +563,13 @@ def main() -> int:
             extras["cone_wide_bp"] = cone_wide_bp()
 
     base = None
-    if args.cpu_sample > 0:
-        log("bench: CPU oracle sample")
-        base = cpu_baseline(geom, phantom.cpu().numpy(), args.cpu_sample)
+    if args.cpu > 0:
+        log("bench: CPU oracle, one full CGLS iteration")
+        del b, phantom
+        torch.cuda.empty_cache()
+        base = cpu_baseline(geom, noisy_np)
 
+    traffic, traffic_state = measured_traffic(dom_name)
     line = {
         "metric": "CGLS iter/s + projector HBM GB/s, cone-beam 512^3",
         "value": args.steps / secs,
@@ -419,7 +595,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom_name),
+            "traffic": traffic,
+            "traffic_source": traffic_state,
             "bytes_alg": bytes_alg,
             "nnz": nnz,
         },
@@ -429,9 +606,11 @@ def main() -> int:
         "bp_gbs": bytes_alg / (bp_ms * 1e-3) / 1e9,
         "rmse_vs_phantom": rmse,
         "norm_atr_last": float(norms[-1]) if len(norms) else None,
+        "accuracy": accuracy,
         "batched": batched,
         "extras": extras,
         "cpu_baseline": base,
+        "kernel_sources": source_stamp(),
     }
     print(json.dumps(line), flush=True)
     return 0

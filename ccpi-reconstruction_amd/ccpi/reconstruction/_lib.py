@@ -17,6 +17,7 @@ CCPI_PARALLEL = 0
 CCPI_CONE = 1
 CCPI_REDUCE_FP32 = 1
 CCPI_TIMING = 2
+CCPI_ACCUM_FP64 = 4
 CCPI_REG_TIKHONOV = 0
 CCPI_REG_TV = 1
 

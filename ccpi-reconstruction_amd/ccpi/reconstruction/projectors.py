@@ -17,8 +17,8 @@ from typing import Sequence
 
 import numpy as np
 
-from ._lib import (CCPI_CONE, CCPI_PARALLEL, CCPI_REDUCE_FP32, CCPI_REG_TIKHONOV,
-                   CCPI_REG_TV, CCPI_TIMING, CcpiGeometry, check, lib)
+from ._lib import (CCPI_ACCUM_FP64, CCPI_CONE, CCPI_PARALLEL, CCPI_REDUCE_FP32,
+                   CCPI_REG_TIKHONOV, CCPI_REG_TV, CCPI_TIMING, CcpiGeometry, check, lib)
 
 
 def _f64(a) -> np.ndarray:
@@ -177,13 +177,16 @@ def sirt(geom: Geometry, b, iterations: int, x=None):
     return x
 
 
-def mlem(geom: Geometry, b, iterations: int, x=None):
-    """MLEM body (mlem.cpp:15-80); x is reset to ones first."""
+def mlem(geom: Geometry, b, iterations: int, x=None, accum_fp64: bool = False):
+    """MLEM body (mlem.cpp:15-80); x is reset to ones first.  With
+    ``accum_fp64`` the projections' sums (A x, A^T (b / A x), A^T 1) are
+    accumulated in double (C ABI flag CCPI_ACCUM_FP64)."""
     import torch
     if x is None:
         x = torch.empty(geom.shape, dtype=torch.float32, device=b.device)
     check(lib().ccpi_mlem(C.byref(geom.c_struct()), _dev(b, geom.sino_shape, "b"),
-                          _dev(x, geom.shape, "x"), int(iterations), 0, _stream()))
+                          _dev(x, geom.shape, "x"), int(iterations),
+                          CCPI_ACCUM_FP64 if accum_fp64 else 0, _stream()))
     return x
 
 

@@ -45,6 +45,10 @@ CONE_512 = dict(n=512, vox=0.5, n_angles=720, n_det=768, pitch=1.0, sod=500.0, s
 CONE_256 = dict(n=256, vox=1.0, n_angles=360, n_det=384, pitch=2.0, sod=500.0, sdd=1000.0)
 CONE_768 = dict(n=768, vox=1.0 / 3.0, n_angles=1200, n_det=1024, pitch=0.75, sod=500.0,
                 sdd=1000.0)
+# the 768^3 precision study's scan at a third of its sampling (same 256 mm
+# volume, 768 mm detector and distances): the size its FP64 oracle arm runs at
+CONE_768_AT_256 = dict(n=256, vox=1.0, n_angles=400, n_det=342, pitch=2.25, sod=500.0,
+                       sdd=1000.0)
 
 
 def ellipsoid_params(n: int, seed: int, dens=(0.1, 1.0)):
@@ -88,6 +92,60 @@ def ellipsoid_phantom(geom: Geometry, n: int, seed: int, dens=(0.1, 1.0), device
         zz = ((cz - c2) / a2) ** 2
         vol += (q2[:, :, None] + zz[None, None, :] <= 1.0).to(torch.float32) * float(d)
     return vol
+
+
+def ellipsoid_phantom_host(geom: Geometry, n: int, seed: int, dens=(0.1, 1.0)) -> np.ndarray:
+    """The phantom of ``ellipsoid_phantom`` built with numpy on the host
+    (float32 [nx][ny][nz]): each ellipsoid is evaluated on its bounding box
+    only, with the same voxel-centre test and float32 additions in the same
+    order.  Host-built inputs are identical on every machine, so results
+    computed elsewhere (committed fixtures) refer to the same phantom."""
+    nx, ny, nz = geom.shape
+    vs, o = geom.voxel_size, geom.origin
+    half = (0.5 * nx * vs[0], 0.5 * ny * vs[1], 0.5 * nz * vs[2])
+    R = min(half[0], half[1])
+    cx = o[0] + (np.arange(nx) + 0.5) * vs[0]
+    cy = o[1] + (np.arange(ny) + 0.5) * vs[1]
+    cz = o[2] + (np.arange(nz) + 0.5) * vs[2]
+    vol = np.zeros((nx, ny, nz), np.float32)
+
+    def span(centres, lo, hi):
+        i0 = int(np.searchsorted(centres, lo, side="left"))
+        i1 = int(np.searchsorted(centres, hi, side="right"))
+        return max(i0 - 1, 0), min(i1 + 1, centres.size)
+
+    for c, ax, rot, d in ellipsoid_params(n, seed, dens):
+        c0, c1, c2 = c[0] * R, c[1] * R, c[2] * half[2]
+        a0, a1, a2 = ax[0] * R, ax[1] * R, ax[2] * half[2]
+        rxy = max(a0, a1)
+        i0, i1 = span(cx, c0 - rxy, c0 + rxy)
+        j0, j1 = span(cy, c1 - rxy, c1 + rxy)
+        k0, k1 = span(cz, c2 - a2, c2 + a2)
+        cr, sr = math.cos(rot), math.sin(rot)
+        X = (cx[i0:i1] - c0)[:, None]
+        Y = (cy[j0:j1] - c1)[None, :]
+        u = (X * cr + Y * sr) / a0
+        w = (-X * sr + Y * cr) / a1
+        q2 = u * u + w * w
+        zz = ((cz[k0:k1] - c2) / a2) ** 2
+        m = q2[:, :, None] + zz[None, None, :] <= 1.0
+        box = vol[i0:i1, j0:j1, k0:k1]
+        box[m] += np.float32(d)
+    return vol
+
+
+def poisson_log_host(sino: np.ndarray, i0: float, seed: int) -> np.ndarray:
+    """Host form of ``poisson_log`` with numpy's PCG64 stream: counts ~
+    Poisson(I0 exp(-b)) drawn in C order (one angle at a time, which draws
+    the same sequence as one call), then -ln(max(counts, 1) / I0) as
+    float32."""
+    rng = np.random.default_rng(seed)
+    out = np.empty(sino.shape, np.float32)
+    for a in range(sino.shape[0]):
+        lam = i0 * np.exp(-sino[a].astype(np.float64))
+        counts = rng.poisson(lam).astype(np.float64)
+        out[a] = -np.log(np.maximum(counts, 1.0) / i0)
+    return out
 
 
 def poisson_log(sino, i0: float, seed: int):

@@ -179,13 +179,13 @@ int ccpi_sirt(const ccpi_geometry *g, const float *b, float *x, int iterations, 
 
 int ccpi_mlem(const ccpi_geometry *g, const float *b, float *x, int iterations, int flags,
               void *stream) {
-  (void)flags;
   return guarded([&] {
+    CCPI_REQUIRE((flags & ~CCPI_ACCUM_FP64) == 0, CCPI_E_ARG, "unknown MLEM flag");
     CCPI_REQUIRE(b && x, CCPI_E_ARG, "null data pointer");
     CCPI_REQUIRE(iterations >= 0, CCPI_E_ARG, "negative iteration count");
     ccpi::Projector P(ccpi::Geometry::from(g), S(stream));
     ccpi::Workspace W;
-    ccpi::run_mlem(P, b, x, iterations, W, S(stream));
+    ccpi::run_mlem(P, b, x, iterations, W, S(stream), (flags & CCPI_ACCUM_FP64) != 0);
     CCPI_HIP(hipStreamSynchronize(S(stream)));
   });
 }

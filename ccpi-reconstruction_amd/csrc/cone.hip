@@ -328,10 +328,13 @@ constexpr int kFpGroup = 8;
 // sum order is the reference's).  The wave stops once the segment start
 // passes every row's exit parameter (amax_bits: a non-negative float's
 // bits, so the integer compare is the float compare).
-template <bool EDGE>
-__device__ inline float fp_run(const FpRow &R, const ConeEnt *__restrict__ e,
-                               const float *__restrict__ ai, const float *vol, int nz,
-                               int mb, int me, int amax_bits, float acc) {
+// Acc is the row accumulator: float (the reference's sum, c2D.cpp:104-143)
+// or double (the FP64-accumulation option: each segment's float
+// contribution is added in double).
+template <bool EDGE, class Acc>
+__device__ inline Acc fp_run(const FpRow &R, const ConeEnt *__restrict__ e,
+                             const float *__restrict__ ai, const float *vol, int nz,
+                             int mb, int me, int amax_bits, Acc acc) {
   int m = mb;
   for (; m + kFpGroup <= me; m += kFpGroup) {
     if (__float_as_int(e[m - 1].alpha) >= amax_bits) return acc;
@@ -343,7 +346,7 @@ __device__ inline float fp_run(const FpRow &R, const ConeEnt *__restrict__ e,
                    c0[j], c1[j], w0[j], w1[j]);
     }
 #pragma unroll
-    for (int j = 0; j < kFpGroup; ++j) acc = acc + (c0[j] * w0[j] + c1[j] * w1[j]);
+    for (int j = 0; j < kFpGroup; ++j) acc = acc + (Acc)(c0[j] * w0[j] + c1[j] * w1[j]);
   }
   for (; m < me; ++m) {
     if (__float_as_int(e[m - 1].alpha) >= amax_bits) return acc;
@@ -351,12 +354,14 @@ __device__ inline float fp_run(const FpRow &R, const ConeEnt *__restrict__ e,
     float c0, c1, w0, w1;
     fp_seg<EDGE>(R, e[m - 1].alpha, en.alpha, ai[m], column_rsrc(vol, en.col, nz), c0, c1,
                  w0, w1);
-    acc = acc + (c0 * w0 + c1 * w1);
+    acc = acc + (Acc)(c0 * w0 + c1 * w1);
   }
   return acc;
 }
 
-// One workgroup per ray and 256 detector rows, one row per lane.
+// One workgroup per ray and 256 detector rows, one row per lane.  Acc as
+// fp_run; with double the result is rounded once, after the d_conv scaling.
+template <class Acc>
 __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
     ConeArgs g, const ConeRay *__restrict__ rays, const ConeEnt *__restrict__ ents,
     const float *__restrict__ eai, const float *__restrict__ vol, float *__restrict__ sino) {
@@ -374,7 +379,7 @@ __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
   const int h = r % g.nh;
   const ConeRay R = rays[r];
   float *out = sino + (size_t)r * g.nv + v;
-  float acc = *out;
+  Acc acc = (Acc)*out;
   // A row whose ray has left the volume's z range at the start of a
   // segment adds nothing from there on (c2D.cpp:153-187 break): the wave
   // stops at the largest such walk parameter of its rows.
@@ -398,14 +403,14 @@ __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
     const float *__restrict__ ai = eai + R.off;
     // segments before ncom: every row uses the truncating form
     const int ncom = min(R.ncom, R.n);
-    acc = fp_run<false>(row, e, ai, vol, nz, 1, ncom, amax_bits, acc);
+    acc = fp_run<false, Acc>(row, e, ai, vol, nz, 1, ncom, amax_bits, acc);
     // from ncom on, rows below min_v use the floor form (c2D.cpp:153-167)
     const bool edge = v < R.min_v;
     row.kb = edge ? g.pzbz1 : g.pzbz;
     row.ks = edge ? 1 : 0;
-    acc = fp_run<true>(row, e, ai, vol, nz, ncom, R.n, amax_bits, acc);
+    acc = fp_run<true, Acc>(row, e, ai, vol, nz, ncom, R.n, amax_bits, acc);
   }
-  if (live) *out = acc * g.dconv[(size_t)h * g.nv + v];
+  if (live) *out = (float)(acc * (Acc)g.dconv[(size_t)h * g.nv + v]);
 }
 
 // eai[q] = ents[q - 1].alpha / voxel z size: the slice-position factor of
@@ -719,10 +724,10 @@ constexpr int bp_min_waves(int S) { return S <= 2 ? 8 : 1; }
 // block below nv is a main row (as in the slot form's main path); otherwise
 // each row takes its row set's form, pass by pass when the sets overlap (as
 // bp_slot).
-template <int S>
+template <int S, class AccT>
 __device__ void bp_phased_block(const ConeArgs &g, const BpRec *__restrict__ R, int n,
                                 const float *__restrict__ dpix, const float *s_vz1,
-                                float *acc1, int vb, int lane, int nph, int ph, float inv_dz,
+                                AccT *acc1, int vb, int lane, int nph, int ph, float inv_dz,
                                 bool main, bool one_set, int min_v, int max_v) {
   constexpr int span = 64 * S;
   const int nz = g.nz, nv = g.nv, mid = g.mid;
@@ -832,12 +837,17 @@ __device__ void bp_phased_block(const ConeArgs &g, const BpRec *__restrict__ R, 
   }
 }
 
-template <int S, bool PHASED>
+// AccT is the type of the LDS column arrays: float (the reference's
+// accumulation, calc_ah_z c2D.cpp:640-751), or double for the FP64-
+// accumulation option, which also leaves out the main-row path (its LDS
+// addressing is float-only) and takes the general row-set path instead.
+template <int S, bool PHASED, class AccT>
 __global__ __launch_bounds__(64 * kBpWaves)
 __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
     ConeArgs g, const BpCol *__restrict__ cols, const BpRec *__restrict__ recs,
     const float *__restrict__ dpix, float *__restrict__ vol) {
   constexpr int narr = PHASED ? 1 : S;  // LDS arrays per wavefront
+  constexpr bool kFloatAcc = std::is_same<AccT, float>::value;
   extern __shared__ float smem[];
   const int nz = g.nz, nv = g.nv;
   // slice planes with plane 0 repeated in front: s_vzx[q + 1] = plane max(q, 0)
@@ -861,9 +871,9 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
   const int colid = ci * g.ny + cj;
   const BpCol C = cols[colid];
   if (C.n == 0) return;  // nothing crosses the column: it stays as it is
-  float *acc = smem + lds_pad4(nz + 2) + wave * narr * astride;
+  AccT *acc = reinterpret_cast<AccT *>(smem + lds_pad4(nz + 2)) + wave * narr * astride;
   float *__restrict__ vcol = vol + (size_t)colid * nz;
-  for (int q = lane; q < narr * astride; q += 64) acc[q] = 0.0f;
+  for (int q = lane; q < narr * astride; q += 64) acc[q] = (AccT)0;
   __builtin_amdgcn_wave_barrier();
   for (int k = lane; k < nz; k += 64) acc[k + 1] = vcol[k];
   __builtin_amdgcn_wave_barrier();
@@ -884,11 +894,11 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
   const int nvp4 = 4 * g.nvp;
   for (int vb = 0; vb < nv; vb += span) {
     if constexpr (PHASED) {
-      bp_phased_block<S>(g, R, C.n, dpix, s_vz1, acc + 1, vb, lane, nph, ph, inv_dz,
+      bp_phased_block<S, AccT>(g, R, C.n, dpix, s_vz1, acc + 1, vb, lane, nph, ph, inv_dz,
                          one_set && g.bp_main_safe && vb >= min_v && min(vb + span, nv) <= max_v,
                          one_set, min_v, max_v);
       continue;
-    } else if (one_set && g.bp_main_safe && vb >= min_v && vb + span <= max_v) {
+    } else if (kFloatAcc && one_set && g.bp_main_safe && vb >= min_v && vb + span <= max_v) {
       // Every row of the block is a main row: its ray stays inside the
       // volume's z range for all of this column's rays (c2D.cpp:657-676), so
       // k is in [0, nz) (the clamp only guards float ties) and k +- 1 in
@@ -902,7 +912,7 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
         const bool lower = v < g.mid;
         dzv[t] = g.dz[v];
         izv[t] = g.iz[v];
-        b0[t] = __builtin_amdgcn_readfirstlane(lds_addr(acc + t * astride + 1));
+        b0[t] = __builtin_amdgcn_readfirstlane(lds_addr(reinterpret_cast<const float *>(acc + t * astride + 1)));
         b1[t] = b0[t] + (lower ? -4 : 4);
         bz[t] = lds_addr(s_vz1 + (lower ? 0 : 1));
       }
@@ -994,7 +1004,7 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
         const float a0 = __builtin_bit_cast(float, rec.y);
         const float a1 = __builtin_bit_cast(float, rec.z);
         const float ai = a0 * inv_dz;
-        float *c0p[S], *c1p[S];
+        AccT *c0p[S], *c1p[S];
         float c0[S], c1[S];
 #pragma unroll
         for (int t = 0; t < S; ++t) {
@@ -1007,14 +1017,14 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
           const float mz = min_sv(a1, az);
           c0[t] = p[t] * (mz - a0);
           c1[t] = p[t] * (a1 - mz);
-          float *ac = acc + t * astride + 1;
+          AccT *ac = acc + t * astride + 1;
           c0p[t] = ac + i0;
           c1p[t] = ac + i1;
         }
         {
           // separate arrays per slot: first-slice updates together, then
           // the second-slice updates
-          float o[S];
+          AccT o[S];
 #pragma unroll
           for (int t = 0; t < S; ++t) o[t] = *c0p[t];
 #pragma unroll
@@ -1056,10 +1066,10 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   for (int k = lane; k < nz; k += 64) {
-    float s = acc[k + 1];
+    AccT s = acc[k + 1];
 #pragma unroll
     for (int t = 1; t < narr; ++t) s += acc[t * astride + k + 1];
-    vcol[k] = s;
+    vcol[k] = (float)s;
   }
 }
 
@@ -1088,10 +1098,10 @@ __global__ void cone_scale_kernel(const float *__restrict__ pix,
 constexpr int kBpMaxStride = 4;
 
 // LDS of one BP workgroup: the slice planes and `narr` column arrays per
-// wavefront.
-static size_t bp_lds_bytes(int nz, int narr) {
+// wavefront of `acc_bytes`-byte cells.
+static size_t bp_lds_bytes(int nz, int narr, size_t acc_bytes = sizeof(float)) {
   const size_t pad = (size_t)((nz + 2 + 3) & ~3);
-  return sizeof(float) * (pad + (size_t)kBpWaves * (size_t)narr * pad);
+  return sizeof(float) * pad + acc_bytes * (size_t)kBpWaves * (size_t)narr * pad;
 }
 
 // BP accumulation layout of a cone geometry (host only, no device work).
@@ -1355,13 +1365,18 @@ void cone_build_plan(ConePlan &P, hipStream_t st) {
   P.args = A;
 }
 
-void cone_fp(const ConePlan &P, const float *vol, float *sino, hipStream_t st) {
+void cone_fp(const ConePlan &P, const float *vol, float *sino, hipStream_t st, bool f64) {
   const ConeArgs &A = P.args;
   dim3 grid((unsigned)(A.na * A.nh), (unsigned)((A.nv + kFpRows - 1) / kFpRows));
   const size_t lds = sizeof(float) * (size_t)(A.nz + 1);
-  hipLaunchKernelGGL(cone_fp_kernel, grid, dim3(kFpRows), lds, st, A,
-                     P.rays.as<ConeRay>(), P.ents.as<ConeEnt>(), P.eai.as<float>(), vol,
-                     sino);
+  if (f64)
+    hipLaunchKernelGGL(cone_fp_kernel<double>, grid, dim3(kFpRows), lds, st, A,
+                       P.rays.as<ConeRay>(), P.ents.as<ConeEnt>(), P.eai.as<float>(), vol,
+                       sino);
+  else
+    hipLaunchKernelGGL(cone_fp_kernel<float>, grid, dim3(kFpRows), lds, st, A,
+                       P.rays.as<ConeRay>(), P.ents.as<ConeEnt>(), P.eai.as<float>(), vol,
+                       sino);
   CCPI_HIP(hipGetLastError());
 }
 
@@ -1388,32 +1403,50 @@ void cone_scale(const ConePlan &P, const float *pix, float *dpix, hipStream_t st
   CCPI_HIP(hipGetLastError());
 }
 
-void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, hipStream_t st) {
+template <class AccT>
+static void cone_bp_launch(const ConePlan &P, const float *dpix, float *vol, hipStream_t st) {
   const ConeArgs &A = P.args;
   // kBpTile x kBpTile column tiles of kBpTile^2 / kBpWaves workgroups
   // (kernel mapping)
   const int blocks = ((A.nx + kBpTile - 1) / kBpTile) * ((A.ny + kBpTile - 1) / kBpTile) *
                      (kBpTile * kBpTile / kBpWaves);
   const int narr = A.bp_phased ? 1 : A.bp_stride;  // LDS arrays per wavefront
-  const size_t lds = bp_lds_bytes(A.nz, narr);
+  const size_t lds = bp_lds_bytes(A.nz, narr, sizeof(AccT));
+  CCPI_REQUIRE(lds <= 160 * 1024, CCPI_E_LIMIT,
+               "cone BP: nz too large for LDS with FP64 accumulation");
   const BpCol *bc = P.bpcol.as<BpCol>();
   const BpRec *br = P.bprec.as<BpRec>();
   const dim3 grid(blocks), block(64 * kBpWaves);
   if (A.bp_phased && A.bp_stride == 4)
-    hipLaunchKernelGGL((cone_bp_kernel<4, true>), grid, block, lds, st, A, bc, br, dpix, vol);
+    hipLaunchKernelGGL((cone_bp_kernel<4, true, AccT>), grid, block, lds, st, A, bc, br, dpix,
+                       vol);
   else if (A.bp_phased && A.bp_stride == 8)
-    hipLaunchKernelGGL((cone_bp_kernel<8, true>), grid, block, lds, st, A, bc, br, dpix, vol);
+    hipLaunchKernelGGL((cone_bp_kernel<8, true, AccT>), grid, block, lds, st, A, bc, br, dpix,
+                       vol);
   else if (A.bp_phased)
-    hipLaunchKernelGGL((cone_bp_kernel<16, true>), grid, block, lds, st, A, bc, br, dpix, vol);
+    hipLaunchKernelGGL((cone_bp_kernel<16, true, AccT>), grid, block, lds, st, A, bc, br, dpix,
+                       vol);
   else if (narr == 1)
-    hipLaunchKernelGGL((cone_bp_kernel<1, false>), grid, block, lds, st, A, bc, br, dpix, vol);
+    hipLaunchKernelGGL((cone_bp_kernel<1, false, AccT>), grid, block, lds, st, A, bc, br, dpix,
+                       vol);
   else if (narr == 2)
-    hipLaunchKernelGGL((cone_bp_kernel<2, false>), grid, block, lds, st, A, bc, br, dpix, vol);
+    hipLaunchKernelGGL((cone_bp_kernel<2, false, AccT>), grid, block, lds, st, A, bc, br, dpix,
+                       vol);
   else if (narr == 3)
-    hipLaunchKernelGGL((cone_bp_kernel<3, false>), grid, block, lds, st, A, bc, br, dpix, vol);
+    hipLaunchKernelGGL((cone_bp_kernel<3, false, AccT>), grid, block, lds, st, A, bc, br, dpix,
+                       vol);
   else
-    hipLaunchKernelGGL((cone_bp_kernel<4, false>), grid, block, lds, st, A, bc, br, dpix, vol);
+    hipLaunchKernelGGL((cone_bp_kernel<4, false, AccT>), grid, block, lds, st, A, bc, br, dpix,
+                       vol);
   CCPI_HIP(hipGetLastError());
+}
+
+void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, hipStream_t st,
+                    bool f64) {
+  if (f64)
+    cone_bp_launch<double>(P, dpix, vol, st);
+  else
+    cone_bp_launch<float>(P, dpix, vol, st);
 }
 
 }  // namespace ccpi

@@ -96,8 +96,9 @@ __device__ inline void par_walk(const ParArgs &g, int a, int h, Add &&add) {
 
 // FP of detector rows [v0, v0 + RC) of ray r (clipped to nv): `acc_in`
 // selects y += A x (reference f2D) or y = A x.  Returns the sum of squares
-// of the rows written.
-template <int RC>
+// of the rows written.  Acc: float row sums (the reference's), or double
+// (the FP64-accumulation option; each float product added in double).
+template <int RC, class Acc = float>
 __device__ inline double par_fp_rows(const ParArgs &g, int r, int v0,
                                      const float *__restrict__ vol,
                                      float *__restrict__ sino, bool acc_in) {
@@ -106,11 +107,11 @@ __device__ inline double par_fp_rows(const ParArgs &g, int r, int v0,
   const int nz = g.nz, ny = g.ny;
   float *__restrict__ out = sino + (size_t)r * g.nv + v0;
   const int nrow = min(RC, g.nv - v0);
-  float acc[RC];
+  Acc acc[RC];
   int kmap[RC];
 #pragma unroll
   for (int q = 0; q < RC; q++) {
-    acc[q] = (acc_in && q < nrow) ? out[q] : 0.f;
+    acc[q] = (acc_in && q < nrow) ? (Acc)out[q] : (Acc)0;
     const int k = q < nrow ? g.map[v0 + q] : -1;
     kmap[q] = (k >= 0 && k < nz) ? k : -1;
   }
@@ -126,28 +127,29 @@ __device__ inline double par_fp_rows(const ParArgs &g, int r, int v0,
         if (4 * q4 + 3 < nval) {
           float4 f;
           __builtin_memcpy(&f, cv + 4 * q4, sizeof(f));
-          acc[4 * q4] = acc[4 * q4] + f.x * l;
-          acc[4 * q4 + 1] = acc[4 * q4 + 1] + f.y * l;
-          acc[4 * q4 + 2] = acc[4 * q4 + 2] + f.z * l;
-          acc[4 * q4 + 3] = acc[4 * q4 + 3] + f.w * l;
+          acc[4 * q4] = acc[4 * q4] + (Acc)(f.x * l);
+          acc[4 * q4 + 1] = acc[4 * q4 + 1] + (Acc)(f.y * l);
+          acc[4 * q4 + 2] = acc[4 * q4 + 2] + (Acc)(f.z * l);
+          acc[4 * q4 + 3] = acc[4 * q4 + 3] + (Acc)(f.w * l);
         } else {
 #pragma unroll
           for (int q = 4 * q4; q < 4 * q4 + 4; q++)
-            if (q < nval) acc[q] = acc[q] + cv[q] * l;
+            if (q < nval) acc[q] = acc[q] + (Acc)(cv[q] * l);
         }
       }
       return;
     }
 #pragma unroll
     for (int q = 0; q < RC; q++)
-      if (kmap[q] >= 0) acc[q] = acc[q] + col[kmap[q]] * l;
+      if (kmap[q] >= 0) acc[q] = acc[q] + (Acc)(col[kmap[q]] * l);
   });
   double ss = 0.0;
 #pragma unroll
   for (int q = 0; q < RC; q++)
     if (q < nrow) {
-      out[q] = acc[q];
-      ss += (double)acc[q] * (double)acc[q];
+      const float o = (float)acc[q];
+      out[q] = o;
+      ss += (double)o * (double)o;
     }
   return ss;
 }
@@ -167,6 +169,7 @@ __device__ inline double par_fp_thread(const ParArgs &g, int bx, int by, int tid
 // ray, each taking 4 consecutive detector rows, so the lanes of a ray read
 // one contiguous run of every crossed column.  Block bx holds rays
 // [bx * 256 / lr, ...), row block by the rows [by * 4 lr, (by + 1) * 4 lr).
+template <class Acc = float>
 __device__ inline double par_fp_lanes(const ParArgs &g, int bx, int by, int tid, int lr,
                                       const float *__restrict__ vol,
                                       float *__restrict__ sino, bool acc_in) {
@@ -174,7 +177,7 @@ __device__ inline double par_fp_lanes(const ParArgs &g, int bx, int by, int tid,
   if (r >= g.na * g.nh) return 0.0;
   const int v0 = (by * lr + (tid & (lr - 1))) * 4;
   if (v0 >= g.nv) return 0.0;
-  return par_fp_rows<4>(g, r, v0, vol, sino, acc_in);
+  return par_fp_rows<4, Acc>(g, r, v0, vol, sino, acc_in);
 }
 
 // Lanes per ray of the FP for nv detector rows: enough for the rows of one
@@ -185,10 +188,89 @@ inline int par_fp_lanes_per_ray(int nv) {
   return lr;
 }
 
+// Footprint of the voxel column [x0, xn] x [y0, yn] on the detector at the
+// angle whose table row is t (p2D.cpp:808-915): detector columns [lo, hi],
+// kind 0 (general: trapezoid over [ymin, ymax]), 1 or 2 (axis-aligned:
+// columns whose +-h lies in [lim0, lim1)).
+struct ParFoot {
+  int lo, hi, kind;
+  double ymin, ymax, lim0, lim1;
+};
+
+__device__ inline ParFoot par_footprint(const ParArgs &g, const double *t, double x0, double xn,
+                                        double y0, double yn) {
+  const double c = t[0], s = t[1];
+  ParFoot f;
+  f.ymin = f.ymax = f.lim0 = f.lim1 = 0.0;
+  if (fabs(c) < kEps) {  // p2D.cpp:808-844
+    f.kind = s < 0.0 ? 1 : 2;
+    f.lim0 = x0;
+    f.lim1 = xn;
+    if (s < 0.0) {
+      f.lo = (int)floor(x0 * g.ihp - g.hp0);
+      f.hi = (int)floor((xn - kEps) * g.ihp - g.hp0);
+    } else {
+      f.lo = (int)ceil((-xn + kEps) * g.ihp - g.hp0);
+      f.hi = (int)ceil(-x0 * g.ihp - g.hp0);
+    }
+    f.lo = max(f.lo, 0);
+    f.hi = min(f.hi, g.nh - 1);
+  } else if (fabs(s) < kEps) {  // p2D.cpp:845-880
+    f.kind = c < 0.0 ? 2 : 1;
+    f.lim0 = y0;
+    f.lim1 = yn;
+    if (c < 0.0) {
+      f.lo = (int)ceil((-yn + kEps) * g.ihp - g.hp0);
+      f.hi = (int)ceil(-y0 * g.ihp - g.hp0);
+    } else {
+      f.lo = (int)floor(y0 * g.ihp - g.hp0);
+      f.hi = (int)floor((yn - kEps) * g.ihp - g.hp0);
+    }
+    f.lo = max(f.lo, 0);
+    f.hi = min(f.hi, g.nh - 1);
+  } else {  // p2D.cpp:881-915
+    f.kind = 0;
+    if (c > 0.0) {
+      if (s > 0.0) { f.ymin = y0 * c - xn * s; f.ymax = yn * c - x0 * s; }
+      else         { f.ymin = y0 * c - x0 * s; f.ymax = yn * c - xn * s; }
+    } else {
+      if (s > 0.0) { f.ymin = yn * c - xn * s; f.ymax = y0 * c - x0 * s; }
+      else         { f.ymin = yn * c - x0 * s; f.ymax = y0 * c - xn * s; }
+    }
+    // clamp at 0: the floor can round to -1 when ymin ~ h_pix[0]
+    f.lo = f.ymin < g.h_pix[0] ? 0 : max((int)floor(f.ymin * g.ihp - g.hp0), 0);
+    f.hi = min((int)floor(f.ymax * g.ihp - g.hp0), g.nh - 1);
+  }
+  return f;
+}
+
+// Weight of detector column position hp in footprint f (p2D.cpp:916-938):
+// 1 with w set, 0 to skip the column, -1 past the footprint's far edge.
+__device__ inline int par_weight(const ParArgs &g, const double *t, const ParFoot &f, double hp,
+                                 double &w) {
+  if (f.kind != 0) {
+    const double tt = f.kind == 1 ? hp : -hp;
+    if (!(tt >= f.lim0 && tt < f.lim1)) return 0;
+    w = fabs(t[0]) < kEps ? g.dy : g.dx;
+    return 1;
+  }
+  const double ybot = f.ymin + t[2], ytop = f.ymax - t[2], l = t[4];
+  if (!(hp > f.ymin + kEps)) return 0;
+  if (hp < ybot)
+    w = l * (hp - f.ymin) * t[3];
+  else if (hp <= ytop)
+    w = l;
+  else if (hp < f.ymax - kEps)
+    w = l * (f.ymax - hp) * t[3];
+  else
+    return -1;
+  return 1;
+}
+
 // One thread of the BP: column bx * 256 + tid, slices [by * RC, by * RC +
 // RC).  `acc_in` selects x += A^T y (reference b2D) or x = A^T y.  Returns
 // the sum of squares of the slices written.
-template <int RC>
+template <int RC, class Acc = float>
 __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid,
                                        const float *__restrict__ sino,
                                        float *__restrict__ vol, bool acc_in) {
@@ -200,9 +282,9 @@ __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid
   const int nz = g.nz, nv = g.nv, nh = g.nh;
   const int nk = min(RC, nz - k0);
   float *__restrict__ vcol = vol + (size_t)colid * nz + k0;
-  float acc[RC];
+  Acc acc[RC];
 #pragma unroll
-  for (int q = 0; q < RC; q++) acc[q] = (acc_in && q < nk) ? vcol[q] : 0.f;
+  for (int q = 0; q < RC; q++) acc[q] = (acc_in && q < nk) ? (Acc)vcol[q] : (Acc)0;
   const double x0 = g.ox + (double)i * g.dx, xn = g.ox + (double)(i + 1) * g.dx;
   const double y0 = g.oy + (double)j * g.dy, yn = g.oy + (double)(j + 1) * g.dy;
   const int T = g.map_type;
@@ -281,14 +363,14 @@ __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid
           if (4 * q4 + 3 < nvalid) {
             float4 f;
             __builtin_memcpy(&f, pr + 4 * q4, sizeof(f));
-            acc[4 * q4] = acc[4 * q4] + f.x * wf;
-            acc[4 * q4 + 1] = acc[4 * q4 + 1] + f.y * wf;
-            acc[4 * q4 + 2] = acc[4 * q4 + 2] + f.z * wf;
-            acc[4 * q4 + 3] = acc[4 * q4 + 3] + f.w * wf;
+            acc[4 * q4] = acc[4 * q4] + (Acc)(f.x * wf);
+            acc[4 * q4 + 1] = acc[4 * q4 + 1] + (Acc)(f.y * wf);
+            acc[4 * q4 + 2] = acc[4 * q4 + 2] + (Acc)(f.z * wf);
+            acc[4 * q4 + 3] = acc[4 * q4 + 3] + (Acc)(f.w * wf);
           } else {
 #pragma unroll
             for (int q = 4 * q4; q < 4 * q4 + 4; q++)
-              if (q < nvalid) acc[q] = acc[q] + pr[q] * wf;
+              if (q < nvalid) acc[q] = acc[q] + (Acc)(pr[q] * wf);
           }
         }
       } else if (T == 2 || T == 4) {  // rows summed before scaling (p2D.cpp:694-761)
@@ -298,7 +380,7 @@ __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid
           if (q < nk && (k + 1) * T <= nv) {
             float sum = p[k * T];
             for (int u = 1; u < T; u++) sum = sum + p[k * T + u];
-            acc[q] = acc[q] + sum * wf;
+            acc[q] = acc[q] + (Acc)(sum * wf);
           }
         }
       } else {
@@ -306,7 +388,7 @@ __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid
         for (int q = 0; q < RC; q++) {
           if (q < nk) {
             const int k = k0 + q;
-            for (int v = g.rlo[k]; v < g.rhi[k]; v++) acc[q] = acc[q] + p[v] * wf;
+            for (int v = g.rlo[k]; v < g.rhi[k]; v++) acc[q] = acc[q] + (Acc)(p[v] * wf);
           }
         }
       }
@@ -316,8 +398,9 @@ __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid
 #pragma unroll
   for (int q = 0; q < RC; q++)
     if (q < nk) {
-      vcol[q] = acc[q];
-      ss += (double)acc[q] * (double)acc[q];
+      const float o = (float)acc[q];
+      vcol[q] = o;
+      ss += (double)o * (double)o;
     }
   return ss;
 }

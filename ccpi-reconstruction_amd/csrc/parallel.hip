@@ -34,16 +34,153 @@ namespace ccpi {
 // 16 needs 121 (4 waves) and was 5 % slower on the 64-problem workload
 constexpr int kRows = 8;
 
+// Acc: float sums (the reference's) or double (FP64-accumulation option)
+template <class Acc>
 __global__ __launch_bounds__(256) void par_fp_kernel(ParArgs g, int lr,
                                                      const float *__restrict__ vol,
                                                      float *__restrict__ sino) {
-  par_fp_lanes(g, blockIdx.x, blockIdx.y, threadIdx.x, lr, vol, sino, true);
+  par_fp_lanes<Acc>(g, blockIdx.x, blockIdx.y, threadIdx.x, lr, vol, sino, true);
 }
 
+template <class Acc>
 __global__ __launch_bounds__(256) void par_bp_kernel(ParArgs g,
                                                      const float *__restrict__ sino,
                                                      float *__restrict__ vol) {
-  par_bp_thread<kRows>(g, blockIdx.x, blockIdx.y, threadIdx.x, sino, vol, true);
+  par_bp_thread<kRows, Acc>(g, blockIdx.x, blockIdx.y, threadIdx.x, sino, vol, true);
+}
+
+// Tiled BP (one row per slice): a workgroup owns a 16 x 16 tile of voxel
+// columns, one column per lane and R consecutive slices per lane.  Angles go
+// in blocks of `ab`: for each angle of the block, the detector columns that
+// the tile's square can reach (its corners' span plus a margin of 2) are
+// staged in LDS with the R rows of the lane's slices (hmax columns at most),
+// one coalesced copy per tile instead of each lane re-reading its columns'
+// rows; every lane then walks its own column's footprint (the reference's
+// FP64 arithmetic, par_footprint / par_weight) and reads the rows from LDS.
+// A footprint column outside the staged span (never expected; rounding
+// guard) is read from global memory.  Angles are visited in order, so with
+// one angle group the result is bit-identical to par_bp_kernel; with G
+// groups (blockIdx.z) each group writes its partial sums of its angle range
+// to part[group] and par_bp_sum_kernel adds them in group order.
+template <int R, class Acc>
+__global__ __launch_bounds__(256) void par_bp_lds_kernel(ParArgs g, int hmax, int ab, int groups,
+                                                         const float *__restrict__ sino,
+                                                         float *__restrict__ vol,
+                                                         float *__restrict__ part) {
+  static_assert(R % 4 == 0, "rows are staged 4 at a time");
+  extern __shared__ float smem[];
+  float *s_rows = smem;                                 // [ab][hmax][R]
+  int *s_lo = reinterpret_cast<int *>(smem + (size_t)ab * hmax * R);
+  int *s_cnt = s_lo + ab;
+  const int tid = threadIdx.x;
+  const int ntj = (g.ny + 15) >> 4;
+  const int ti0 = (int)(blockIdx.x / ntj) * 16, tj0 = (int)(blockIdx.x % ntj) * 16;
+  const int i = ti0 + (tid >> 4), j = tj0 + (tid & 15);
+  const bool active = i < g.nx && j < g.ny;
+  const int k0 = blockIdx.y * R;
+  const int nz = g.nz, nv = g.nv, nh = g.nh;
+  const int nk = min(R, nz - k0);
+  const int grp = blockIdx.z;
+  const int a_begin = (int)((long long)g.na * grp / groups);
+  const int a_end = (int)((long long)g.na * (grp + 1) / groups);
+  const size_t colid = (size_t)(active ? i : 0) * g.ny + (active ? j : 0);
+  Acc acc[R];
+  float *__restrict__ vcol = vol + colid * nz + k0;
+#pragma unroll
+  for (int q = 0; q < R; q++)
+    acc[q] = (groups == 1 && active && q < nk) ? (Acc)vcol[q] : (Acc)0;
+  const double x0 = g.ox + (double)i * g.dx, xn = g.ox + (double)(i + 1) * g.dx;
+  const double y0 = g.oy + (double)j * g.dy, yn = g.oy + (double)(j + 1) * g.dy;
+  // tile square
+  const double X0 = g.ox + (double)ti0 * g.dx, XN = g.ox + (double)min(ti0 + 16, g.nx) * g.dx;
+  const double Y0 = g.oy + (double)tj0 * g.dy, YN = g.oy + (double)min(tj0 + 16, g.ny) * g.dy;
+  // rows of the lane's slices that exist (one row per slice)
+  const int nrow = max(0, min(R, min(nv, nz) - k0));
+  constexpr int Q4 = R / 4;
+  for (int a0 = a_begin; a0 < a_end; a0 += ab) {
+    const int nblk = min(ab, a_end - a0);
+    if (tid < nblk) {
+      const double *t = g.atab + 5 * (a0 + tid);
+      const double c = t[0], s = t[1];
+      const double p0 = Y0 * c - X0 * s, p1 = YN * c - X0 * s;
+      const double p2 = Y0 * c - XN * s, p3 = YN * c - XN * s;
+      const double tmin = fmin(fmin(p0, p1), fmin(p2, p3));
+      const double tmax = fmax(fmax(p0, p1), fmax(p2, p3));
+      const int lo = max((int)floor(tmin * g.ihp - g.hp0) - 2, 0);
+      const int hi = min((int)floor(tmax * g.ihp - g.hp0) + 2, nh - 1);
+      s_lo[tid] = lo;
+      s_cnt[tid] = max(0, min(hi - lo + 1, hmax));
+    }
+    __syncthreads();
+    const int total = nblk * hmax * Q4;
+    for (int e = tid; e < total; e += 256) {
+      const int q4 = e % Q4, r = e / Q4;
+      const int hh = r % hmax, t = r / hmax;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (hh < s_cnt[t]) {
+        const float *p = sino + ((size_t)(a0 + t) * nh + s_lo[t] + hh) * nv + k0 + 4 * q4;
+        if (4 * q4 + 3 < nrow) {
+          __builtin_memcpy(&v, p, sizeof(v));
+        } else {
+          if (4 * q4 < nrow) v.x = p[0];
+          if (4 * q4 + 1 < nrow) v.y = p[1];
+          if (4 * q4 + 2 < nrow) v.z = p[2];
+        }
+      }
+      *reinterpret_cast<float4 *>(s_rows + ((size_t)t * hmax + hh) * R + 4 * q4) = v;
+    }
+    __syncthreads();
+    if (active) {
+      for (int t = 0; t < nblk; t++) {
+        const int a = a0 + t;
+        const double *tab = g.atab + 5 * a;
+        const ParFoot f = par_footprint(g, tab, x0, xn, y0, yn);
+        const int slo = s_lo[t], scnt = s_cnt[t];
+        for (int h = f.lo; h <= f.hi; h++) {
+          double w;
+          const int use = par_weight(g, tab, f, g.h_pix[h], w);
+          if (use == 0) continue;
+          if (use < 0) break;
+          const float wf = (float)w;
+          const int hh = h - slo;
+          if (hh >= 0 && hh < scnt) {
+            const float *pr = s_rows + ((size_t)t * hmax + hh) * R;
+#pragma unroll
+            for (int q4 = 0; q4 < Q4; q4++) {
+              const float4 v = *reinterpret_cast<const float4 *>(pr + 4 * q4);
+              acc[4 * q4] = acc[4 * q4] + (Acc)(v.x * wf);
+              acc[4 * q4 + 1] = acc[4 * q4 + 1] + (Acc)(v.y * wf);
+              acc[4 * q4 + 2] = acc[4 * q4 + 2] + (Acc)(v.z * wf);
+              acc[4 * q4 + 3] = acc[4 * q4 + 3] + (Acc)(v.w * wf);
+            }
+          } else {
+            const float *pr = sino + ((size_t)a * nh + h) * nv + k0;
+#pragma unroll
+            for (int q = 0; q < R; q++)
+              if (q < nrow) acc[q] = acc[q] + (Acc)(pr[q] * wf);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!active) return;
+  float *__restrict__ dst = groups == 1 ? vcol : part + (size_t)grp * g.nx * g.ny * nz + colid * nz + k0;
+#pragma unroll
+  for (int q = 0; q < R; q++)
+    if (q < nk) dst[q] = (float)acc[q];
+}
+
+// vol += sum of the angle groups' partials, in group order (in Acc).
+template <class Acc>
+__global__ void par_bp_sum_kernel(const float *__restrict__ part, float *__restrict__ vol,
+                                  size_t n, int groups) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+    Acc s = (Acc)part[q];
+    for (int p = 1; p < groups; p++) s = s + (Acc)part[(size_t)p * n + q];
+    vol[q] = (float)((Acc)vol[q] + s);
+  }
 }
 
 // Nonzero system-matrix entries: per ray, crossed cells with nonzero xy
@@ -111,9 +248,15 @@ void par_nnz(const ParPlan &P, unsigned long long *count, hipStream_t st) {
   CCPI_HIP(hipGetLastError());
 }
 
-void par_build_plan(ParPlan &P, hipStream_t st) {
-  const Geometry &G = P.g;
-  ParArgs A{};
+// Host tables of one parallel geometry and the scalars of its argument
+// block (table pointers left for the caller to set).
+struct ParTables {
+  std::vector<double> atab;
+  std::vector<int> map, rlo, rhi;
+};
+
+static ParTables par_tables(const Geometry &G, ParArgs &A) {
+  A = ParArgs{};
   A.na = G.na; A.nh = G.nh; A.nv = G.nv; A.nx = G.nx; A.ny = G.ny; A.nz = G.nz;
   A.ox = G.origin[0]; A.oy = G.origin[1];
   A.dx = G.vsize[0]; A.dy = G.vsize[1];
@@ -122,15 +265,19 @@ void par_build_plan(ParPlan &P, hipStream_t st) {
   A.dconv = (double)(float)(3.0 * A.D);
   A.ihp = 1.0 / (G.h_pix[1] - G.h_pix[0]);
   A.hp0 = G.h_pix[0] / (G.h_pix[1] - G.h_pix[0]);
+  ParTables T;
   // row -> slice map and its pattern (p2D.cpp:22-61)
-  std::vector<int> map(G.nv), rlo(G.nz, 0), rhi(G.nz, 0);
+  std::vector<int> &map = T.map, &rlo = T.rlo, &rhi = T.rhi;
+  map.resize(G.nv);
+  rlo.assign(G.nz, 0);
+  rhi.assign(G.nz, 0);
   for (int v = 0; v < G.nv; v++)
     map[v] = (int)std::floor((G.v_pix[v] - G.origin[2]) / G.vsize[2]);
   A.map_type = 0;
-  for (int T = 1; T <= 4; T *= 2) {
+  for (int ppv = 1; ppv <= 4; ppv *= 2) {
     bool ok = true;
-    for (int v = 0; v < G.nv && ok; v++) ok = map[v] == v / T;
-    if (ok) { A.map_type = T; break; }
+    for (int v = 0; v < G.nv && ok; v++) ok = map[v] == v / ppv;
+    if (ok) { A.map_type = ppv; break; }
   }
   for (int k = 0; k < G.nz; k++) {
     int lo = G.nv, hi = G.nv;
@@ -142,7 +289,8 @@ void par_build_plan(ParPlan &P, hipStream_t st) {
     rhi[k] = std::max(lo, hi);
   }
   // per-angle trapezoid parameters (p2D.cpp:996-1049)
-  std::vector<double> atab((size_t)G.na * 5);
+  std::vector<double> &atab = T.atab;
+  atab.resize((size_t)G.na * 5);
   for (int a = 0; a < G.na; a++) {
     const double c = std::cos(G.angles[a]), s = std::sin(G.angles[a]);
     const double *o = G.origin, *d = G.vsize;
@@ -165,11 +313,47 @@ void par_build_plan(ParPlan &P, hipStream_t st) {
     t[3] = 1.0 / (ybot - ymin);
     t[4] = std::fabs(c) > std::fabs(s) ? d[0] / std::fabs(c) : d[1] / std::fabs(s);
   }
-  P.atab.upload(atab, st);
+  return T;
+}
+
+// LDS of one tiled-BP workgroup: the staged rows and the per-angle spans.
+static size_t par_bp_lds_bytes(const ParBpShape &S) {
+  return sizeof(float) * (size_t)S.ab * S.hmax * S.slices + 2 * sizeof(int) * (size_t)S.ab;
+}
+
+// Shape of the tiled BP for a geometry, or tiled = 0.  The staged span of a
+// 16 x 16 tile is its corners' detector span plus 2 columns each side; in
+// column-index units (the first pixel step, as the footprints use) it is at
+// most the tile's diagonal times 1 / step, so hmax bounds it with a margin.
+// Angle groups are added while the tiles alone give fewer than 4096 waves
+// (four per SIMD on 256 CUs).
+static ParBpShape par_bp_shape(const Geometry &G, const ParArgs &A) {
+  ParBpShape S;
+  if (A.map_type != 1 || G.nv % 4 != 0) return S;
+  S.slices = G.nz <= 8 ? 8 : 16;
+  const double span = std::hypot(16.0 * G.vsize[0], 16.0 * G.vsize[1]) * std::fabs(A.ihp);
+  if (!(span < 4096.0)) return S;
+  S.hmax = (int)std::ceil(span) + 6;
+  const size_t per_angle = sizeof(float) * (size_t)S.hmax * S.slices + 2 * sizeof(int);
+  if (per_angle > 32 * 1024) return S;
+  S.ab = (int)std::max<size_t>(1, std::min<size_t>(8, (32 * 1024) / per_angle));
+  const long long tiles = (long long)((G.nx + 15) / 16) * ((G.ny + 15) / 16);
+  const long long waves = tiles * ((G.nz + S.slices - 1) / S.slices) * 4;
+  S.groups = (int)std::min<long long>(std::min<long long>(8, G.na),
+                                      std::max<long long>(1, (4096 + waves - 1) / waves));
+  S.tiled = 1;
+  return S;
+}
+
+void par_build_plan(ParPlan &P, hipStream_t st) {
+  const Geometry &G = P.g;
+  ParArgs A;
+  const ParTables T = par_tables(G, A);
+  P.atab.upload(T.atab, st);
   P.hpix.upload(G.h_pix, st);
-  P.map.upload(map, st);
-  P.rlo.upload(rlo, st);
-  P.rhi.upload(rhi, st);
+  P.map.upload(T.map, st);
+  P.rlo.upload(T.rlo, st);
+  P.rhi.upload(T.rhi, st);
   A.atab = P.atab.as<double>();
   A.h_pix = P.hpix.as<double>();
   A.map = P.map.as<int>();
@@ -177,23 +361,63 @@ void par_build_plan(ParPlan &P, hipStream_t st) {
   A.rhi = P.rhi.as<int>();
   CCPI_HIP(hipStreamSynchronize(st));
   P.args = A;
+  P.bp = par_bp_shape(G, A);
 }
 
-void par_fp(const ParPlan &P, const float *vol, float *sino, hipStream_t st) {
+void par_fp(const ParPlan &P, const float *vol, float *sino, hipStream_t st, bool f64) {
   const ParArgs &A = P.args;
   const int rays = A.na * A.nh;
   const int lr = par_fp_lanes_per_ray(A.nv), rpb = 256 / lr;
   dim3 grid((unsigned)((rays + rpb - 1) / rpb), (unsigned)((A.nv + 4 * lr - 1) / (4 * lr)));
-  hipLaunchKernelGGL(par_fp_kernel, grid, dim3(256), 0, st, A, lr, vol, sino);
+  if (f64)
+    hipLaunchKernelGGL(par_fp_kernel<double>, grid, dim3(256), 0, st, A, lr, vol, sino);
+  else
+    hipLaunchKernelGGL(par_fp_kernel<float>, grid, dim3(256), 0, st, A, lr, vol, sino);
   CCPI_HIP(hipGetLastError());
 }
 
-void par_bp(const ParPlan &P, const float *sino, float *vol, hipStream_t st) {
+size_t par_bp_scratch_floats(const ParPlan &P) {
+  return P.bp.tiled && P.bp.groups > 1 ? (size_t)P.bp.groups * P.g.n_vox() : 0;
+}
+
+template <class Acc>
+static void par_bp_launch(const ParPlan &P, const float *sino, float *vol, float *scratch,
+                          hipStream_t st) {
   const ParArgs &A = P.args;
-  const int cols = A.nx * A.ny;
-  dim3 grid((unsigned)((cols + 255) / 256), (unsigned)((A.nz + kRows - 1) / kRows));
-  hipLaunchKernelGGL(par_bp_kernel, grid, dim3(256), 0, st, A, sino, vol);
+  const ParBpShape &S = P.bp;
+  if (!S.tiled) {
+    const int cols = A.nx * A.ny;
+    dim3 grid((unsigned)((cols + 255) / 256), (unsigned)((A.nz + kRows - 1) / kRows));
+    hipLaunchKernelGGL(par_bp_kernel<Acc>, grid, dim3(256), 0, st, A, sino, vol);
+    CCPI_HIP(hipGetLastError());
+    return;
+  }
+  CCPI_REQUIRE(S.groups == 1 || scratch, CCPI_E_ARG, "parallel BP: angle groups need scratch");
+  const unsigned tiles = (unsigned)(((A.nx + 15) / 16) * ((A.ny + 15) / 16));
+  const dim3 grid(tiles, (unsigned)((A.nz + S.slices - 1) / S.slices), (unsigned)S.groups);
+  const size_t lds = par_bp_lds_bytes(S);
+  if (S.slices == 8)
+    hipLaunchKernelGGL((par_bp_lds_kernel<8, Acc>), grid, dim3(256), lds, st, A, S.hmax, S.ab,
+                       S.groups, sino, vol, scratch);
+  else
+    hipLaunchKernelGGL((par_bp_lds_kernel<16, Acc>), grid, dim3(256), lds, st, A, S.hmax,
+                       S.ab, S.groups, sino, vol, scratch);
   CCPI_HIP(hipGetLastError());
+  if (S.groups > 1) {
+    const size_t n = P.g.n_vox();
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(par_bp_sum_kernel<Acc>, dim3(blocks), dim3(256), 0, st, scratch, vol,
+                       n, S.groups);
+    CCPI_HIP(hipGetLastError());
+  }
+}
+
+void par_bp(const ParPlan &P, const float *sino, float *vol, float *scratch, hipStream_t st,
+            bool f64) {
+  if (f64)
+    par_bp_launch<double>(P, sino, vol, scratch, st);
+  else
+    par_bp_launch<float>(P, sino, vol, scratch, st);
 }
 
 }  // namespace ccpi

@@ -264,42 +264,47 @@ Projector::Projector(const Geometry &g, hipStream_t st) {
     par = par_plan(g, st);
 }
 
-void Projector::fp(const float *vol, float *sino, hipStream_t st) const {
+void Projector::fp(const float *vol, float *sino, hipStream_t st, bool f64) const {
   if (cone)
-    cone_fp(*cone, vol, sino, st);
+    cone_fp(*cone, vol, sino, st, f64);
   else
-    par_fp(*par, vol, sino, st);
+    par_fp(*par, vol, sino, st, f64);
 }
 
-size_t Projector::bp_scratch_floats() const { return cone ? cone_dpix_floats(*cone) : 0; }
+size_t Projector::bp_scratch_floats() const {
+  return cone ? cone_dpix_floats(*cone) : par_bp_scratch_floats(*par);
+}
 
 void Projector::bp_scaled(const float *dpix, float *vol, hipStream_t st) const {
   CCPI_REQUIRE(cone, CCPI_E_ARG, "pre-scaled backprojection is a cone-beam form");
   cone_bp_scaled(*cone, dpix, vol, st);
 }
 
-void Projector::bp(const float *sino, float *vol, hipStream_t st, float *scratch) const {
-  if (!cone) {
-    par_bp(*par, sino, vol, st);
+void Projector::bp(const float *sino, float *vol, hipStream_t st, float *scratch,
+                   bool f64) const {
+  auto run = [&](float *scr) {
+    if (cone) {
+      cone_scale(*cone, sino, scr, st);
+      cone_bp_scaled(*cone, scr, vol, st, f64);
+    } else {
+      par_bp(*par, sino, vol, scr, st, f64);
+    }
+  };
+  const size_t need = bp_scratch_floats();
+  if (scratch || need == 0) {
+    run(scratch);
     return;
   }
-  const ConePlan &P = *cone;
-  if (scratch) {
-    cone_scale(P, sino, scratch, st);
-    cone_bp_scaled(P, scratch, vol, st);
-    return;
-  }
-  SharedScratch &S = const_cast<SharedScratch &>(P.scratch);
+  SharedScratch &S = const_cast<SharedScratch &>(cone ? cone->scratch : par->scratch);
   std::lock_guard<std::mutex> lk(S.mu);
   if (!S.done) CCPI_HIP(hipEventCreateWithFlags(&S.done, hipEventDisableTiming));
   if (!S.buf.p) {
-    S.buf.alloc(cone_dpix_floats(P) * sizeof(float));
+    S.buf.alloc(need * sizeof(float));
   } else {
-    // the previous caller's BP (any stream) has finished reading the copy
+    // the previous caller's BP (any stream) has finished with the buffer
     CCPI_HIP(hipStreamWaitEvent(st, S.done, 0));
   }
-  cone_scale(P, sino, S.buf.as<float>(), st);
-  cone_bp_scaled(P, S.buf.as<float>(), vol, st);
+  run(S.buf.as<float>());
   CCPI_HIP(hipEventRecord(S.done, st));
 }
 

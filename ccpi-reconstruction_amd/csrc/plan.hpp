@@ -112,12 +112,26 @@ struct ConePlan {
   }
 };
 
+// Launch shape of the tiled parallel BP (detector rows staged in LDS per
+// 16 x 16 column tile and block of angles); tiled = 0 keeps the
+// column-per-lane kernel (rows per slice other than 1, unaligned rows, or
+// footprints too wide for LDS).
+struct ParBpShape {
+  int tiled = 0;
+  int slices = 16;   // slices per lane (8 or 16)
+  int hmax = 0;      // detector columns staged per angle
+  int ab = 8;        // angles per staged block
+  int groups = 1;    // angle groups summed through scratch partials
+};
+
 struct ParPlan {
   Geometry g;
   ParArgs args{};
+  ParBpShape bp{};
   DeviceBuffer atab, hpix, map, rlo, rhi;
+  SharedScratch scratch;  // BP partials of callers without a workspace
   size_t bytes() const {
-    return atab.bytes + hpix.bytes + map.bytes + rlo.bytes + rhi.bytes;
+    return atab.bytes + hpix.bytes + map.bytes + rlo.bytes + rhi.bytes + scratch.bytes();
   }
 };
 
@@ -136,15 +150,23 @@ BpLayout cone_bp_layout(const Geometry &g);
 
 // launchers (cone.hip, parallel.hip)
 void cone_build_plan(ConePlan &P, hipStream_t st);
-void cone_fp(const ConePlan &P, const float *vol, float *sino, hipStream_t st);
+// f64: accumulate the projector sums in double (the FP64-accumulation
+// option of the solvers); the default is the reference's float sums
+void cone_fp(const ConePlan &P, const float *vol, float *sino, hipStream_t st,
+             bool f64 = false);
 // dpix holds na * nh rows of args.nvp floats (cone_dpix_floats)
 size_t cone_dpix_floats(const ConePlan &P);
 void cone_scale(const ConePlan &P, const float *pix, float *dpix, hipStream_t st);
-void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, hipStream_t st);
+void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, hipStream_t st,
+                    bool f64 = false);
 
 void par_build_plan(ParPlan &P, hipStream_t st);
-void par_fp(const ParPlan &P, const float *vol, float *sino, hipStream_t st);
-void par_bp(const ParPlan &P, const float *sino, float *vol, hipStream_t st);
+void par_fp(const ParPlan &P, const float *vol, float *sino, hipStream_t st, bool f64 = false);
+// scratch: par_bp_scratch_floats(P) floats (partials of the angle groups),
+// unused when that is 0
+void par_bp(const ParPlan &P, const float *sino, float *vol, float *scratch, hipStream_t st,
+            bool f64 = false);
+size_t par_bp_scratch_floats(const ParPlan &P);
 // nonzero system-matrix entries (gathered voxel operands of one FP)
 void cone_nnz(const ConePlan &P, unsigned long long *count, hipStream_t st);
 void par_nnz(const ParPlan &P, unsigned long long *count, hipStream_t st);
@@ -155,11 +177,14 @@ struct Projector {
   std::shared_ptr<ConePlan> cone;
   std::shared_ptr<ParPlan> par;
   explicit Projector(const Geometry &g, hipStream_t st);
-  void fp(const float *vol, float *sino, hipStream_t st) const;
+  // f64: projector sums accumulated in double (the solvers' FP64-
+  // accumulation option); default: the reference's float sums
+  void fp(const float *vol, float *sino, hipStream_t st, bool f64 = false) const;
   // scratch: bp_scratch_floats() floats owned by the caller and used by
   // no other call meanwhile (a solver's workspace); nullptr takes the
   // plan's shared scratch
-  void bp(const float *sino, float *vol, hipStream_t st, float *scratch = nullptr) const;
+  void bp(const float *sino, float *vol, hipStream_t st, float *scratch = nullptr,
+          bool f64 = false) const;
   // BP of an already scaled sinogram in the padded BP layout (cone only):
   // a solver's fused update kernels write it
   void bp_scaled(const float *dpix, float *vol, hipStream_t st) const;

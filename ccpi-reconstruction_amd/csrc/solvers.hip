@@ -482,8 +482,11 @@ void run_sirt(const Projector &P, const float *b, float *x, int iters, Workspace
   CCPI_HIP(hipGetLastError());
 }
 
+// f64: the projections' sums (A x and the ratio backprojection A^T (b / A x),
+// and the sensitivity A^T 1) accumulated in double, each rounded to float
+// once; the element-wise steps are unchanged.
 void run_mlem(const Projector &P, const float *b, float *x, int iters, Workspace &W,
-              hipStream_t st) {
+              hipStream_t st, bool f64) {
   const size_t nvox = P.geometry().n_vox(), npix = P.geometry().n_pix();
   W.ensure(P);
   float *N = W.v1.as<float>(), *t = W.v2.as<float>();
@@ -492,14 +495,14 @@ void run_mlem(const Projector &P, const float *b, float *x, int iters, Workspace
   dev_fill(x, nvox, 1.f, st);
   dev_fill(r, npix, 1.f, st);
   dev_fill(N, nvox, 0.f, st);
-  P.bp(r, N, st, W.bp_scratch());
+  P.bp(r, N, st, W.bp_scratch(), f64);
   hipLaunchKernelGGL(invert_nz_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, N, nvox);
   for (int it = 0; it < iters; it++) {
     dev_fill(r, npix, 0.f, st);
-    P.fp(x, r, st);
+    P.fp(x, r, st, f64);
     hipLaunchKernelGGL(mlem_ratio_kernel, dim3(grid_for(npix)), dim3(256), 0, st, r, b, npix);
     dev_fill(t, nvox, 0.f, st);
-    P.bp(r, t, st, W.bp_scratch());
+    P.bp(r, t, st, W.bp_scratch(), f64);
     hipLaunchKernelGGL(mlem_update_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, x, N, t,
                        nvox);
   }

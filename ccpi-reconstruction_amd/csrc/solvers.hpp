@@ -76,7 +76,7 @@ void run_cgls_reg(const Projector &P, float *b, float *x, int iters, int kind, d
 void run_sirt(const Projector &P, const float *b, float *x, int iters,
               Workspace &W, hipStream_t st);
 void run_mlem(const Projector &P, const float *b, float *x, int iters,
-              Workspace &W, hipStream_t st);
+              Workspace &W, hipStream_t st, bool f64 = false);
 unsigned long long projector_nnz(const Projector &P, hipStream_t st);
 // batched.hip: parallel-beam problems of uneven sizes, one persistent launch
 // per step, iteration replayed from a hipGraph (FP64 reductions)

@@ -111,7 +111,12 @@ int ccpi_cgls(const ccpi_geometry *g, float *b, float *x, int iterations,
 int ccpi_sirt(const ccpi_geometry *g, const float *b, float *x,
               int iterations, int flags, void *stream);
 
-/* MLEM (Core/src/Algorithms/mlem.cpp:15-80); x is reset to 1 first. */
+/* MLEM (Core/src/Algorithms/mlem.cpp:15-80); x is reset to 1 first.
+   flags: 0 (the reference's float sums) or CCPI_ACCUM_FP64: the
+   projections' sums - A x, the ratio backprojection A^T (b / A x) and the
+   sensitivity A^T 1 - accumulate in double and are rounded to float once
+   (projector weights and element-wise steps stay float). */
+#define CCPI_ACCUM_FP64      4
 int ccpi_mlem(const ccpi_geometry *g, const float *b, float *x,
               int iterations, int flags, void *stream);
 

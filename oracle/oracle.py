@@ -71,6 +71,9 @@ def lib():
             f = getattr(_lib, f"or_cgls_{sfx}")
             f.argtypes = [C.POINTER(_Geom), tp, tp, C.c_int, dp]
             f.restype = C.c_int
+            f = getattr(_lib, f"or_cgls_track_{sfx}")
+            f.argtypes = [C.POINTER(_Geom), tp, tp, C.c_int, dp, C.POINTER(C.c_float), dp, dp]
+            f.restype = C.c_int
             for name in ("or_sirt", "or_mlem"):
                 f = getattr(_lib, f"{name}_{sfx}")
                 f.argtypes = [C.POINTER(_Geom), tp, tp, C.c_int]
@@ -233,6 +236,26 @@ def cgls(geom: Geometry, sino, niter, x=None, dtype=np.float32):
     g = geom._struct()
     getattr(lib(), f"or_cgls_{sfx}")(C.byref(g), _ptr(b, ct), _ptr(x, ct), niter, _dp(norms))
     return x, b, norms[:niter]
+
+
+def cgls_track(geom: Geometry, sino, niter, truth=None, dtype=np.float32, timing=False):
+    """CGLS body from x = 0 that also records RMSE(x - truth) after every
+    iteration (truth given) and each iteration's wall time; returns (x,
+    residual, norms, rmse, seconds per iteration)."""
+    sfx, ct = _sfx(dtype)
+    b = np.array(sino, dtype=dtype, order="C", copy=True)
+    x = np.zeros(geom.shape, dtype)
+    norms = np.zeros(max(niter, 1), np.float64)
+    rmse = np.zeros(max(niter, 1), np.float64)
+    secs = np.zeros(max(niter, 1), np.float64)
+    tp = None
+    if truth is not None:
+        t = np.ascontiguousarray(truth, dtype=np.float32)
+        tp = _ptr(t, C.c_float)
+    g = geom._struct()
+    getattr(lib(), f"or_cgls_track_{sfx}")(C.byref(g), _ptr(b, ct), _ptr(x, ct), niter,
+                                           _dp(norms), tp, _dp(rmse), _dp(secs))
+    return x, b, norms[:niter], rmse[:niter], secs[:niter]
 
 
 def sirt(geom: Geometry, sino, niter, x=None, dtype=np.float32):

@@ -1002,11 +1002,30 @@ static RT FN(sumsq)(const RT *p, long n0, long n1, long n2)
   return tot;
 }
 
+/* root mean square of x - truth over n values, in double (fixed order per
+   thread block, a double atomic sum: only the last bits vary with the
+   thread count) */
+static double FN(rmse)(const RT *x, const float *truth, long n)
+{
+  double tot = 0.0;
+  long q;
+#pragma omp parallel for schedule(static) reduction(+ : tot)
+  for (q = 0; q < n; q++) {
+    const double e = (double)x[q] - (double)truth[q];
+    tot += e * e;
+  }
+  return sqrt(tot / (double)n);
+}
+
 /* CGLS as cgls_base::reconstruct + cgls_3d (cgls.cpp:26-157).  b is the
    instrument's sinogram and is overwritten with the residual; x receives
    x + CGLS(b) (x is not used to form the first residual).  norms[it] gets
-   sqrt(||A^T r||^2) like norm_r. */
-int FN(or_cgls)(const or_geom *g, RT *b, RT *x, int niter, double *norms)
+   sqrt(||A^T r||^2) like norm_r.  With truth and rmse given (may be NULL),
+   rmse[it] = RMSE(x - truth) after iteration it; iter_s (may be NULL)
+   receives each iteration's wall time (projections and vector updates,
+   without the RMSE). */
+int FN(or_cgls_track)(const or_geom *g, RT *b, RT *x, int niter, double *norms,
+                      const float *truth, double *rmse, double *iter_s)
 {
   const long nvox = (long)g->nx * g->ny * g->nz;
   const long npix = (long)g->na * g->nh * g->nv;
@@ -1021,6 +1040,7 @@ int FN(or_cgls)(const or_geom *g, RT *b, RT *x, int niter, double *norms)
   for (it = 0; it < niter; it++) {
     RT alpha;
     double nn, beta;
+    const double t_start = omp_get_wtime();
     memset(Ad, 0, sizeof(RT) * (size_t)npix);
     FN(fp)(g, d, Ad);
     alpha = FN(sumsq)(Ad, g->na, g->nh, g->nv);
@@ -1044,11 +1064,20 @@ int FN(or_cgls)(const or_geom *g, RT *b, RT *x, int niter, double *norms)
     }
     if (norms)
       norms[it] = sqrt((double)normr2);
+    if (iter_s)
+      iter_s[it] = omp_get_wtime() - t_start;
+    if (truth && rmse)
+      rmse[it] = FN(rmse)(x, truth, nvox);
   }
   free(d);
   free(s);
   free(Ad);
   return 0;
+}
+
+int FN(or_cgls)(const or_geom *g, RT *b, RT *x, int niter, double *norms)
+{
+  return FN(or_cgls_track)(g, b, x, niter, norms, NULL, NULL, NULL);
 }
 
 /* SIRT (sirt.cpp:15-80): R = 1/(A 1), C = 1/(A^T 1) with entries <= 0 set
@@ -1255,6 +1284,7 @@ int FN(or_cgls_reg)(const or_geom *g, RT *b, RT *x, int niter, int kind,
   for (it = 0; it < niter; it++) {
     RT alpha;
     double nn, beta;
+    const double t_start = omp_get_wtime();
     memset(Ad, 0, sizeof(RT) * (size_t)npix);
     FN(fp)(g, d, Ad);
     if (kind == 0) FN(or_tikhonov)(d, L, g->nx, g->ny, g->nz);

@@ -1,0 +1,167 @@
+"""The cone-beam CGLS precision study at its workload size (GPU test):
+768^3 volume of 1/3 mm, 1200 projections over 360 deg, 1024^2 detector of
+0.75 mm, SOD 500, SDD 1000, 50-ellipsoid phantom (seed 5, densities
+U[0, 0.05] /mm), Poisson noise I0 = 5e3.
+
+Projector properties at this size, where the oracle would take hours:
+  * A 1 equals the closed-form chord of every ray through the volume box on
+    four whole projections (within 5e-6 of the longest chord);
+  * the adjoint identity (FP64 dot products) within 1e-5;
+  * exact scaling A(2x) = 2 A(x), A^T(2y) = 2 A^T y, and run-to-run
+    identical backprojections.
+
+CGLS for 100 iterations with FP64 reductions (default) and with FP32
+reductions, RMSE against the phantom and ||A^T r|| per iteration (printed
+every 10 iterations with -s):
+  * ||r|| never increases (CGLS minimises it over growing Krylov spaces;
+    4e-6 relative slack for the float residual updates);
+  * the two variants agree to 1e-4 in RMSE over the first 10 iterations
+    (before the noise is being fitted) and to 1 % at iteration 100, and both
+    reach the same RMSE minimum to 0.1 %.
+
+The study's FP64-oracle arm runs the same scan at a third of its sampling
+(256^3 of 1 mm, 400 projections, 342^2 detector of 2.25 mm): the oracle's
+FP64 and float (the reference's arithmetic) CGLS, 100 iterations, are
+committed in tests/golden/accuracy_cone256x100.npz (made by
+tests/golden/make_accuracy.py).  The device builds the same data (the
+noise-free sinogram's SHA-256 must match: the cone FP is bit-identical to
+the model) and runs both reduction variants:
+  * over the first 10 iterations the RMSE equals the FP64 oracle's to 1e-6
+    (the two oracles agree to 1e-7 there) and ||A^T r|| to 1e-5;
+  * the RMSE minimum (semi-convergence) falls at the oracle's iteration with
+    the same value to 1e-5;
+  * at iteration 100, where float rounding has been amplified by the noise
+    fitting, the FP64-reduction RMSE is no further from the FP64 oracle's
+    than the reference's float arithmetic is.
+"""
+
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import bruteforce as bf
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def cone768():
+    from ccpi.reconstruction import workloads as W
+    return W.cone_geometry(**W.CONE_768)
+
+
+def test_768_chords_adjoint_scaling_determinism(cone768):
+    from ccpi.reconstruction.projectors import backward_project, forward_project
+    g = cone768
+    print("\n768^3 FP of ones", flush=True)
+    y = forward_project(g, torch.ones(g.shape, device="cuda"))
+    lo = tuple(g.origin)
+    hi = tuple(g.origin[q] + g.shape[q] * g.voxel_size[q] for q in range(3))
+    worst = 0.0
+    for a in (0, 151, 600, 1033):
+        want = bf.cone_box_lengths(g.source, g.detector_x, g.angles[a], g.h_pixels,
+                                   g.v_pixels, [(lo, hi)])
+        got = y[a].double().cpu().numpy()
+        worst = max(worst, float(np.abs(got - want).max() / want.max()))
+    print(f"max deviation from the chord {worst:.2e} of the longest chord", flush=True)
+    assert worst < 5e-6
+    del y
+    gen = torch.Generator(device="cuda").manual_seed(51)
+    x = torch.rand(g.shape, device="cuda", generator=gen)
+    s = torch.rand(g.sino_shape, device="cuda", generator=gen)
+    print("768^3 adjoint", flush=True)
+    y = forward_project(g, x)
+    b = backward_project(g, s)
+    lhs = float(torch.sum(y.double() * s.double()))
+    rhs = float(torch.sum(x.double() * b.double()))
+    print(f"<Ax,y> {lhs:.9e} <x,A^T y> {rhs:.9e}", flush=True)
+    assert abs(lhs - rhs) < 1e-5 * abs(lhs)
+    assert torch.equal(forward_project(g, 2 * x), 2 * y)
+    del y
+    assert torch.equal(backward_project(g, 2 * s), 2 * b)
+    assert torch.equal(backward_project(g, s), b)
+
+
+def test_768_cgls_100_iterations_fp64_vs_fp32_reductions(cone768):
+    from ccpi.reconstruction import workloads as W
+    from ccpi.reconstruction.projectors import CglsSession, forward_project
+    g = cone768
+    iters = 100
+    ph = torch.from_numpy(W.ellipsoid_phantom_host(g, 50, seed=5, dens=(0.0, 0.05))).cuda()
+    b = W.poisson_log(forward_project(g, ph), 5e3, seed=5)
+    runs = {}
+    for name, fp32 in (("fp64-reductions", False), ("fp32-reductions", True)):
+        x = torch.zeros(g.shape, dtype=torch.float32, device="cuda")
+        r = b.clone()
+        s = CglsSession(g, r, x, iters, reduce_fp32=fp32)
+        rmse, rnorm = [], []
+        for it in range(iters):
+            s.iterate(1)
+            rmse.append(float(torch.sqrt(torch.mean((x - ph).double() ** 2))))
+            rnorm.append(float(torch.linalg.vector_norm(r.double())))
+            if (it + 1) % 10 == 0:
+                print(f"{name} it {it + 1}: rmse {rmse[-1]:.6e} ||r|| {rnorm[-1]:.6e}",
+                      flush=True)
+        norms = s.norms()
+        s.close()
+        runs[name] = (np.array(rmse), np.array(rnorm), norms)
+        del x, r
+    for name, (rmse, rnorm, _) in runs.items():
+        assert np.all(np.diff(rnorm) <= 4e-6 * rnorm[:-1]), name
+    a, b32 = runs["fp64-reductions"][0], runs["fp32-reductions"][0]
+    assert np.all(np.abs(a[:10] / b32[:10] - 1) < 1e-4)
+    assert abs(a[-1] / b32[-1] - 1) < 1e-2
+    assert abs(a.min() / b32.min() - 1) < 1e-3
+    print(f"RMSE minimum {a.min():.6e} at iteration {int(a.argmin()) + 1} (fp64 reductions), "
+          f"{b32.min():.6e} at {int(b32.argmin()) + 1} (fp32)", flush=True)
+
+
+def _run_tracked(g, b, ph, iters, fp32):
+    from ccpi.reconstruction.projectors import CglsSession
+    x = torch.zeros(g.shape, dtype=torch.float32, device="cuda")
+    s = CglsSession(g, b.clone(), x, iters, reduce_fp32=fp32)
+    rmse = []
+    for _ in range(iters):
+        s.iterate(1)
+        rmse.append(float(torch.sqrt(torch.mean((x - ph).double() ** 2))))
+    norms = s.norms()
+    s.close()
+    return np.array(rmse), np.asarray(norms), x
+
+
+def test_256_arm_against_the_oracle_runs():
+    from ccpi.reconstruction import workloads as W
+    from ccpi.reconstruction.projectors import forward_project
+    f = np.load(os.path.join(GOLDEN, "accuracy_cone256x100.npz"))
+    g = W.cone_geometry(**W.CONE_768_AT_256)
+    ph_np = W.ellipsoid_phantom_host(g, 50, seed=5, dens=(0.0, 0.05))
+    ph = torch.from_numpy(ph_np).cuda()
+    clean = forward_project(g, ph).cpu().numpy()
+    assert hashlib.sha256(clean.tobytes()).hexdigest() == str(f["sha256_clean"])
+    bn = W.poisson_log_host(clean, 5e3, seed=5)
+    assert hashlib.sha256(bn.tobytes()).hexdigest() == str(f["sha256_noisy"])
+    b = torch.from_numpy(bn).cuda()
+    r64, n64 = f["rmse_f64"], f["norms_f64"]
+    r32 = f["rmse_f32"]
+    dev64, dn64, x = _run_tracked(g, b, ph, 100, fp32=False)
+    dev32, _, _ = _run_tracked(g, b, ph, 100, fp32=True)
+    for it in (0, 9, 12, 19, 49, 99):
+        print(f"it {it + 1}: rmse fp64 oracle {r64[it]:.6e} float oracle {r32[it]:.6e} "
+              f"device fp64-red {dev64[it]:.6e} fp32-red {dev32[it]:.6e}", flush=True)
+    assert np.all(np.abs(dev64[:10] / r64[:10] - 1) < 1e-6)
+    assert np.all(np.abs(dn64[:10] / n64[:10] - 1) < 1e-5)
+    assert int(dev64.argmin()) == int(r64.argmin())
+    assert abs(dev64.min() / r64.min() - 1) < 1e-5
+    assert abs(dev64[-1] - r64[-1]) <= abs(r32[-1] - r64[-1])
+    sub = 4
+    xs = x[::sub, ::sub, ::sub].cpu().numpy().astype(np.float64)
+    x64 = f["x_f64_sub"].astype(np.float64)
+    e_dev = np.linalg.norm(xs - x64) / np.linalg.norm(x64)
+    e_ref = np.linalg.norm(f["x_f32_sub"].astype(np.float64) - x64) / np.linalg.norm(x64)
+    print(f"iterate 100 distance to FP64 oracle: device {e_dev:.3e}, float oracle {e_ref:.3e}",
+          flush=True)

@@ -130,3 +130,4 @@ def test_find_centre_rejects_bad_arguments():
     g.detector_x = 50.0
     assert L.ccpi_xtek_find_centre(C.byref(g), dummy, 5, C.byref(y), None) == -1
     assert b"v_slice" in L.ccpi_last_error()
+
