@@ -3,6 +3,7 @@ the 768^3 scale, and the differentiable TV-regularised reconstruction.
 
   python -m ccpi.reconstruction.studies precision [iterations]
   python -m ccpi.reconstruction.studies tv [steps]
+  python -m ccpi.reconstruction.studies run <workload> [iterations] [--copy-calibration]
 
 Each prints one JSON object per line (per-iteration records, then a
 summary).
@@ -92,12 +93,64 @@ def tv(steps: int = 30, lam: float = 1e-3):
            "nnz": projector_nnz(g)})
 
 
+def run(workload: str, iters: int = 5, calibrate: bool = False):
+    """A few CGLS iterations of one workload, for kernel traces and counter
+    runs (rocprofv3) that should see one shape per kernel:
+      cone512   the 512^3 cone scan (720 x 768^2);
+      wide      the same scan through the XTek volume rule (768^3 of
+                0.811 mm, the phased BP);
+      cfg1      parallel 256 x 256 x 16, 180 angles, 367 x 16 detector;
+      batch64   the 64 uneven parallel problems, plain (one at a time);
+      batch64b  the same problems, batched (one persistent launch per step).
+    ``calibrate`` first copies 1 GiB device to device (the known-size kernel
+    that scales FETCH_SIZE / WRITE_SIZE counters)."""
+    import numpy as np
+    import torch
+
+    from . import instruments as ins
+    from .projectors import cgls, cgls_batched
+    if calibrate:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
+    if workload in ("cone512", "wide", "cfg1"):
+        if workload == "cone512":
+            g = W.cone_geometry(**W.CONE_512)
+        elif workload == "wide":
+            ang = np.linspace(0, 360, 720, endpoint=False).astype(np.float32)
+            g = ins.xtek_setup((720, 768, 768), ang, 1, -500.0, 500.0, 1.0, 1.0, 128.0).geometry
+        else:
+            g = W.parallel_geometry(256, 16, 180, 367)
+        gen = torch.Generator(device="cuda").manual_seed(7)
+        b = torch.rand(g.sino_shape, device="cuda", generator=gen)
+        cgls(g, b, iters)
+    elif workload in ("batch64", "batch64b"):
+        geoms = W.batched_problems(64, seed=3)
+        gen = torch.Generator(device="cuda").manual_seed(7)
+        bs = [torch.rand(g.sino_shape, device="cuda", generator=gen) for g in geoms]
+        if workload == "batch64b":
+            cgls_batched(geoms, bs, iters)
+        else:
+            for g, b in zip(geoms, bs):
+                cgls(g, b, iters)
+    else:
+        raise SystemExit(f"unknown workload {workload}")
+    torch.cuda.synchronize()
+    _emit({"summary": "run", "workload": workload, "iterations": iters})
+
+
 if __name__ == "__main__":
     which = sys.argv[1] if len(sys.argv) > 1 else "precision"
+    if which == "run":
+        run(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 5,
+            calibrate="--copy-calibration" in sys.argv)
+        sys.exit(0)
     n = int(sys.argv[2]) if len(sys.argv) > 2 else 30
     if which == "precision":
         precision(n)
     elif which == "tv":
         tv(n)
     else:
-        raise SystemExit("usage: python -m ccpi.reconstruction.studies precision|tv [n]")
+        raise SystemExit("usage: python -m ccpi.reconstruction.studies "
+                         "precision|tv [n] | run <workload> [iterations]")

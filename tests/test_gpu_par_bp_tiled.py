@@ -27,7 +27,7 @@ CASES = {
     "ragged_chunks": PB((37, 21, 20), (1.0, 1.0, 1.0),
                         np.deg2rad(np.r_[0.0, 45.0, 90.0, np.linspace(3, 177, 27)]),
                         centred(56, 1.0, 0.31), centred(24, 1.0)),
-    "fine_pixels": PB((24, 30, 12), (1.0, 1.2, 1.0), np.deg2rad(np.linspace(0, 180, 40,
+    "fine_pixels": PB((24, 30, 12), (1.0, 1.0, 1.0), np.deg2rad(np.linspace(0, 180, 40,
                                                                            endpoint=False)),
                       centred(110, 0.35, -0.2), centred(12, 1.0)),
     "r8_groups": PB((16, 16, 4), (1.0, 1.0, 1.0), np.deg2rad(np.linspace(0, 180, 64,
