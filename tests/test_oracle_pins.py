@@ -207,3 +207,34 @@ def test_source_height_fp_exact_bp_transpose_outside_end_slices(sz):
         assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
     else:
         assert abs(lhs - rhs) > 0.1 * abs(lhs)
+
+
+@pytest.mark.parametrize("dy", [1.0, 1.2])
+def test_parallel_bp_flat_top_rule_for_rectangular_voxels(dy):
+    """The parallel b2D's trapezoid plateau is vox_size[0] / |cos| when
+    |cos| > |sin|, else vox_size[1] / |sin| (p2D.cpp:1045-1048).  The
+    longest chord of a dx x dy rectangle is min(dx / |cos|, dy / |sin|),
+    which that rule picks only when dx == dy: with square voxels the FP64
+    oracle's b2D is the exact transpose of the brute-force matrix, with
+    dy = 1.2 dx it departs from it (and from the f2D, whose walk lengths
+    are exact) by half a percent in the adjoint identity.  The device BP
+    reproduces the rule (tests/test_gpu_par_bp_tiled.py compares it with
+    the oracle; its adjoint checks use square voxels)."""
+    import bruteforce as bf
+    g = PB((6, 5, 2), (1.0, dy, 1.0), np.deg2rad([10.0, 35.0, 50.0, 80.0, 125.0, 160.0]),
+           centred(14, 0.7, 0.1), centred(2, 1.0))
+    og = oracle_geom(g)
+    A = bf.parallel_matrix(g.angles, g.h_pix, g.v_pix, g.origin, g.vsize, g.shape)
+    rng = np.random.default_rng(14)
+    y = rng.random(g.sino_shape)
+    x = rng.random(g.shape)
+    bp = og.bp(y).ravel()
+    want = A.T @ y.ravel()
+    lhs = np.vdot(og.fp(x), y)
+    rhs = np.vdot(x, og.bp(y))
+    if dy == 1.0:
+        assert np.abs(bp - want).max() <= 1e-9 * np.abs(want).max()
+        assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
+    else:
+        assert np.abs(bp - want).max() > 1e-3 * np.abs(want).max()
+        assert abs(lhs - rhs) > 1e-3 * abs(lhs)
