@@ -198,7 +198,8 @@ def batched_vs_plain(iters: int, small: bool = False):
     import torch
 
     from ccpi.reconstruction import workloads as W
-    from ccpi.reconstruction.projectors import CglsSession, cgls_batched, forward_project
+    from ccpi.reconstruction.projectors import (CglsSession, cgls_batched, forward_project,
+                                                projector_nnz)
 
     geoms = (W.batched_problems(256, seed=5, sizes=(64, 128), slices=(1, 4), angles=(90, 180))
              if small else W.batched_problems(64, seed=3))
@@ -224,7 +225,7 @@ def batched_vs_plain(iters: int, small: bool = False):
     sess = []
     for g, b in zip(geoms, bs):
         x = torch.zeros(g.shape, dtype=torch.float32, device=b.device)
-        sess.append((CglsSession(g, b.clone(), x, iters + 1), x))
+        sess.append((CglsSession(g, b.clone(), x, iters + 1, timing=True), x))
     torch.cuda.synchronize()
     sess[0][0].iterate(1)                # warm the one-problem kernels
     torch.cuda.synchronize()
@@ -244,6 +245,22 @@ def batched_vs_plain(iters: int, small: bool = False):
     for ph, (ss, x2) in zip(phs[1:], sess[1:]):
         rm_p.append(float(torch.sqrt(torch.mean((x2 - ph) ** 2))))
     rel = max(abs(a - b) / max(b, 1e-30) for a, b in zip(rm_b[1:], rm_p))
+    # the plain path's projector launches (HIP events of every session) and
+    # the mix's algorithmic bytes: its projector throughput
+    fp_ms = bp_ms = 0.0
+    n_fp = n_bp = 0
+    bytes_alg = 0
+    for (ss, _), g in zip(sess, geoms):
+        f, nf, bk, nb = ss.timings()
+        fp_ms += f
+        bp_ms += bk
+        n_fp += nf
+        n_bp += nb
+        nnz = projector_nnz(g)
+        bytes_alg += 4 * nnz + 4 * (int(np.prod(g.shape)) + int(np.prod(g.sino_shape)))
+    launches = len(geoms)
+    fp_it = fp_ms / max(n_fp // launches, 1)
+    bp_it = bp_ms / max(n_bp // launches, 1)
     for ss, _ in sess:
         ss.close()
     return {
@@ -258,6 +275,11 @@ def batched_vs_plain(iters: int, small: bool = False):
         "speedup": per_p / per_b,
         "rmse_mean": float(np.mean(rm_b)),
         "rmse_max_rel_diff_batched_vs_plain": rel,
+        "plain_projectors": {
+            "fp_ms_per_iter": fp_it, "bp_ms_per_iter": bp_it, "bytes_alg": bytes_alg,
+            "fp_frac": bytes_alg / (fp_it * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "bp_frac": bytes_alg / (bp_it * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        },
     }
 
 

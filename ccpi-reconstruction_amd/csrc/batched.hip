@@ -47,6 +47,8 @@ struct BatchProb {
   float alpha, beta;
   int it;
   int fp_lr;           // FP lanes per ray
+  int bp_tiled;        // BP items are 16 x 16 column tiles staged in LDS
+  int bp_hmax, bp_ab;  // their staged detector columns per angle, angles per block
 };
 
 struct BatchLists {
@@ -77,36 +79,57 @@ __device__ inline double wave_sum_d(double v) {
 }
 
 // FP of d into Ad for every problem (mode 0), or BP of r into s (mode 1),
-// or BP of r into d (mode 2, initial direction).  Work is taken per
+// or BP of r into d (mode 2, initial direction).  FP work is taken per
 // wavefront: wave item w is quarter w % 4 (64 threads) of block item w / 4,
-// so no wave waits at a barrier for the longest walk of its workgroup; each
-// wave item leaves one FP64 partial norm.
+// so no wave waits at a barrier for the longest walk of its workgroup.  BP
+// work is taken per workgroup: a 16 x 16 column tile whose detector rows
+// the workgroup stages in LDS (par_bp_tile, the one-problem path's tiled
+// body) or, for problems the tiled form does not take, 256 columns of the
+// column-per-lane body.  Every wave of an item leaves one FP64 partial norm.
 template <int RC, int MODE>
 __global__ __launch_bounds__(256) void batch_proj_kernel(BatchProb *__restrict__ P,
                                                          BatchLists L) {
   const int *off = MODE == 0 ? L.fp_off : L.bp_off;
   double *part = MODE == 0 ? L.part_fp : L.part_bp;
-  const int total = 4 * off[L.count];
   const int lane = threadIdx.x & 63;
-  const int wid = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  for (int w = wid; w < total; w += 4 * gridDim.x) {
-    const int item = w >> 2;
-    const int tid = (w & 3) * 64 + lane;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  if (MODE == 0) {
+    const int total = 4 * off[L.count];
+    const int wid = blockIdx.x * 4 + wave;
+    for (int w = wid; w < total; w += 4 * gridDim.x) {
+      const int item = w >> 2;
+      const int tid = (w & 3) * 64 + lane;
+      const int p = find_prob(off, L.count, item);
+      const BatchProb &B = P[p];
+      // the problem's geometry as a local value (scalar registers), as the
+      // one-problem kernels get it through their arguments
+      const ParArgs g = B.g;
+      const int local = item - off[p];
+      double ss =
+          par_fp_lanes(g, local % B.fp_nbx, local / B.fp_nbx, tid, B.fp_lr, B.d, B.Ad, false);
+      ss = wave_sum_d(ss);
+      if (lane == 0) part[w] = ss;
+    }
+    return;
+  }
+  extern __shared__ float smem[];
+  const int total = off[L.count];
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
     const int p = find_prob(off, L.count, item);
     const BatchProb &B = P[p];
-    // the problem's geometry as a local value (scalar registers), as the
-    // one-problem kernels get it through their arguments
     const ParArgs g = B.g;
     const int local = item - off[p];
+    float *dst = MODE == 1 ? B.s : B.d;
     double ss;
-    if (MODE == 0) {
-      ss = par_fp_lanes(g, local % B.fp_nbx, local / B.fp_nbx, tid, B.fp_lr, B.d, B.Ad, false);
+    if (B.bp_tiled) {
+      ss = par_bp_tile<RC, float>(g, local % B.bp_nbx, local / B.bp_nbx, 0, g.na, B.bp_hmax,
+                                  B.bp_ab, false, B.r, dst, smem);
     } else {
-      ss = par_bp_thread<RC>(g, local % B.bp_nbx, local / B.bp_nbx, tid, B.r,
-                             MODE == 1 ? B.s : B.d, false);
+      ss = par_bp_thread<RC>(g, local % B.bp_nbx, local / B.bp_nbx, threadIdx.x, B.r, dst,
+                             false);
     }
     ss = wave_sum_d(ss);
-    if (lane == 0) part[w] = ss;
+    if (lane == 0) part[4 * item + wave] = ss;
   }
 }
 
@@ -190,36 +213,38 @@ int cu_count();
 // once: the work items are dealt out by a fixed stride, so a workgroup that
 // only starts when another one ends would run its share at the tail.
 template <class K>
-int blocks_per_cu(K kernel) {
+int blocks_per_cu(K kernel, size_t lds) {
   int per_cu = 0;
-  CCPI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
+  CCPI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds));
   return std::max(per_cu, 1);
 }
 
+// lds: dynamic LDS of the BP launches (the tiled items' staged rows)
 template <int RC, int MODE>
-void launch_one(int grid, BatchProb *P, const BatchLists &L, hipStream_t st) {
-  static const int per_cu = blocks_per_cu(batch_proj_kernel<RC, MODE>);
+void launch_one(int grid, size_t lds, BatchProb *P, const BatchLists &L, hipStream_t st) {
+  const size_t l = MODE == 0 ? 0 : lds;
+  const int per_cu = blocks_per_cu(batch_proj_kernel<RC, MODE>, l);
   const int g = std::max(1, std::min(grid, per_cu * cu_count()));
-  hipLaunchKernelGGL((batch_proj_kernel<RC, MODE>), dim3(g), dim3(256), 0, st, P, L);
+  hipLaunchKernelGGL((batch_proj_kernel<RC, MODE>), dim3(g), dim3(256), l, st, P, L);
 }
 
 template <int RC>
-void launch_proj(int mode, int grid, BatchProb *P, const BatchLists &L, hipStream_t st) {
+void launch_proj(int mode, int grid, size_t lds, BatchProb *P, const BatchLists &L,
+                 hipStream_t st) {
   if (mode == 0)
-    launch_one<RC, 0>(grid, P, L, st);
+    launch_one<RC, 0>(grid, lds, P, L, st);
   else if (mode == 1)
-    launch_one<RC, 1>(grid, P, L, st);
+    launch_one<RC, 1>(grid, lds, P, L, st);
   else
-    launch_one<RC, 2>(grid, P, L, st);
+    launch_one<RC, 2>(grid, lds, P, L, st);
 }
 
-void proj(int rc, int mode, int grid, BatchProb *P, const BatchLists &L, hipStream_t st) {
+void proj(int rc, int mode, int grid, size_t lds, BatchProb *P, const BatchLists &L,
+          hipStream_t st) {
   if (rc == 8)
-    launch_proj<8>(mode, grid, P, L, st);
-  else if (rc == 32)
-    launch_proj<32>(mode, grid, P, L, st);
+    launch_proj<8>(mode, grid, lds, P, L, st);
   else
-    launch_proj<16>(mode, grid, P, L, st);
+    launch_proj<16>(mode, grid, lds, P, L, st);
   CCPI_HIP(hipGetLastError());
 }
 
@@ -230,18 +255,32 @@ struct BatchRun {
   BatchLists L{};
   int count = 0;
   int fp_total = 0, bp_total = 0;
+  size_t bp_lds = 0;   // dynamic LDS of the BP launches
 };
 
-// Item lists for row/slice blocks of `rc`.
+// Item lists for row/slice blocks of `rc`.  A problem whose plan takes the
+// tiled BP gets tile items of rc slices, its staged span from the plan and
+// as many angles per staged block as fit 32 KB.
 void layout(BatchRun &R, int rc, hipStream_t st) {
   std::vector<int> off(4 * (R.count + 1), 0);
+  R.bp_lds = 0;
   for (int p = 0; p < R.count; p++) {
     BatchProb &B = R.host[p];
     const ParArgs &g = B.g;
+    const ParBpShape &S = R.plans[p]->bp;
     B.fp_lr = par_fp_lanes_per_ray(g.nv);
     B.fp_nbx = (g.na * g.nh + 256 / B.fp_lr - 1) / (256 / B.fp_lr);
     B.fp_nby = (g.nv + 4 * B.fp_lr - 1) / (4 * B.fp_lr);
-    B.bp_nbx = (g.nx * g.ny + 255) / 256;
+    B.bp_tiled = S.tiled;
+    if (S.tiled) {
+      const size_t per_angle = sizeof(float) * (size_t)S.hmax * rc + 2 * sizeof(int);
+      B.bp_hmax = S.hmax;
+      B.bp_ab = (int)std::max<size_t>(1, std::min<size_t>(8, (32 * 1024) / per_angle));
+      R.bp_lds = std::max(R.bp_lds, per_angle * B.bp_ab);
+      B.bp_nbx = ((g.nx + 15) / 16) * ((g.ny + 15) / 16);
+    } else {
+      B.bp_nbx = (g.nx * g.ny + 255) / 256;
+    }
     B.bp_nby = (g.nz + rc - 1) / rc;
     off[p + 1] = off[p] + B.fp_nbx * B.fp_nby;
     off[(R.count + 1) + p + 1] = off[(R.count + 1) + p] + B.bp_nbx * B.bp_nby;
@@ -323,7 +362,7 @@ void run_pb_cgls_batched(const std::vector<Geometry> &geoms, float *const *b,
   if (const char *env = std::getenv("CCPI_BATCH_TUNE")) {
     // "rc,blocks_per_cu" pins the launch shape (skips the timing sweep)
     int rc = 16, mult = 4;
-    if (std::sscanf(env, "%d,%d", &rc, &mult) == 2 && (rc == 8 || rc == 16 || rc == 32) &&
+    if (std::sscanf(env, "%d,%d", &rc, &mult) == 2 && (rc == 8 || rc == 16) &&
         mult > 0) {
       T = Tuned{rc, mult * cu_count()};
       have = true;
@@ -342,16 +381,16 @@ void run_pb_cgls_batched(const std::vector<Geometry> &geoms, float *const *b,
     CCPI_HIP(hipEventCreate(&e0));
     CCPI_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    for (int rc : {8, 16, 32}) {
+    for (int rc : {8, 16}) {
       layout(R, rc, st);
       for (int mult : {2, 4, 8}) {
         const int grid = mult * cu_count();
         // FP of d (scratch, contents irrelevant) and BP of r into s: the
         // two projector launches of an iteration, outputs discarded
-        proj(rc, 0, grid, R.probs.as<BatchProb>(), R.L, st);
+        proj(rc, 0, grid, R.bp_lds, R.probs.as<BatchProb>(), R.L, st);
         CCPI_HIP(hipEventRecord(e0, st));
-        proj(rc, 0, grid, R.probs.as<BatchProb>(), R.L, st);
-        proj(rc, 1, grid, R.probs.as<BatchProb>(), R.L, st);
+        proj(rc, 0, grid, R.bp_lds, R.probs.as<BatchProb>(), R.L, st);
+        proj(rc, 1, grid, R.bp_lds, R.probs.as<BatchProb>(), R.L, st);
         CCPI_HIP(hipEventRecord(e1, st));
         CCPI_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -372,7 +411,7 @@ void run_pb_cgls_batched(const std::vector<Geometry> &geoms, float *const *b,
   const int vgrid = 4 * cu_count();
 
   // start: d = A^T b, gamma = ||d||^2 (cgls.cpp:45-49)
-  proj(T.rc, 2, T.grid, P, R.L, st);
+  proj(T.rc, 2, T.grid, R.bp_lds, P, R.L, st);
   hipLaunchKernelGGL(batch_scalar_kernel<0>, dim3(R.count), dim3(256), 0, st, P, R.L);
   CCPI_HIP(hipGetLastError());
 
@@ -380,10 +419,10 @@ void run_pb_cgls_batched(const std::vector<Geometry> &geoms, float *const *b,
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   CCPI_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  proj(T.rc, 0, T.grid, P, R.L, st);
+  proj(T.rc, 0, T.grid, R.bp_lds, P, R.L, st);
   hipLaunchKernelGGL(batch_scalar_kernel<1>, dim3(R.count), dim3(256), 0, st, P, R.L);
   hipLaunchKernelGGL(batch_vec_kernel<0>, dim3(vgrid), dim3(256), 0, st, P, R.L);
-  proj(T.rc, 1, T.grid, P, R.L, st);
+  proj(T.rc, 1, T.grid, R.bp_lds, P, R.L, st);
   hipLaunchKernelGGL(batch_scalar_kernel<2>, dim3(R.count), dim3(256), 0, st, P, R.L);
   hipLaunchKernelGGL(batch_vec_kernel<1>, dim3(vgrid), dim3(256), 0, st, P, R.L);
   CCPI_HIP(hipStreamEndCapture(st, &graph));

@@ -405,4 +405,131 @@ __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid
   return ss;
 }
 
+// Tiled BP (one row per slice): a workgroup owns a 16 x 16 tile of voxel
+// columns, one column per lane and R consecutive slices per lane.  Angles go
+// in blocks of `ab`: for each angle of the block, the detector columns that
+// the tile's square can reach (its corners' span plus a margin of 2) are
+// staged in LDS with the R rows of the lane's slices (hmax columns at most),
+// one coalesced copy per tile instead of each lane re-reading its columns'
+// rows; every lane then walks its own column's footprint (the reference's
+// FP64 arithmetic, par_footprint / par_weight) and reads the rows from LDS.
+// A footprint column outside the staged span (never expected; rounding
+// guard) is read from global memory.  Angles are visited in order, so with
+// one angle group the result is bit-identical to par_bp_kernel; with G
+// groups (blockIdx.z) each group writes its partial sums of its angle range
+// to part[group] and par_bp_sum_kernel adds them in group order.
+// One tile item of the tiled BP, run by all 256 threads of a workgroup
+// (it synchronises the workgroup): tile `tile` (16 x 16 columns, row-major
+// over the column grid), slices [zc * R, zc * R + R), angles [a_begin,
+// a_end).  The result goes to `dst` (a volume-layout array), added to its
+// contents when acc_in.  Returns the thread's sum of squares of the values
+// it wrote (0 for threads outside the volume).  smem: par_bp_lds_bytes.
+template <int R, class Acc>
+__device__ double par_bp_tile(const ParArgs &g, int tile, int zc, int a_begin, int a_end,
+                              int hmax, int ab, bool acc_in, const float *__restrict__ sino,
+                              float *__restrict__ dst_vol, float *smem) {
+  static_assert(R % 4 == 0, "rows are staged 4 at a time");
+  float *s_rows = smem;                                 // [ab][hmax][R]
+  int *s_lo = reinterpret_cast<int *>(smem + (size_t)ab * hmax * R);
+  int *s_cnt = s_lo + ab;
+  const int tid = threadIdx.x;
+  const int ntj = (g.ny + 15) >> 4;
+  const int ti0 = (tile / ntj) * 16, tj0 = (tile % ntj) * 16;
+  const int i = ti0 + (tid >> 4), j = tj0 + (tid & 15);
+  const bool active = i < g.nx && j < g.ny;
+  const int k0 = zc * R;
+  const int nz = g.nz, nv = g.nv, nh = g.nh;
+  const int nk = min(R, nz - k0);
+  const size_t colid = (size_t)(active ? i : 0) * g.ny + (active ? j : 0);
+  Acc acc[R];
+  float *__restrict__ vcol = dst_vol + colid * nz + k0;
+#pragma unroll
+  for (int q = 0; q < R; q++) acc[q] = (acc_in && active && q < nk) ? (Acc)vcol[q] : (Acc)0;
+  const double x0 = g.ox + (double)i * g.dx, xn = g.ox + (double)(i + 1) * g.dx;
+  const double y0 = g.oy + (double)j * g.dy, yn = g.oy + (double)(j + 1) * g.dy;
+  // tile square
+  const double X0 = g.ox + (double)ti0 * g.dx, XN = g.ox + (double)min(ti0 + 16, g.nx) * g.dx;
+  const double Y0 = g.oy + (double)tj0 * g.dy, YN = g.oy + (double)min(tj0 + 16, g.ny) * g.dy;
+  // rows of the lane's slices that exist (one row per slice)
+  const int nrow = max(0, min(R, min(nv, nz) - k0));
+  constexpr int Q4 = R / 4;
+  for (int a0 = a_begin; a0 < a_end; a0 += ab) {
+    const int nblk = min(ab, a_end - a0);
+    if (tid < nblk) {
+      const double *t = g.atab + 5 * (a0 + tid);
+      const double c = t[0], s = t[1];
+      const double p0 = Y0 * c - X0 * s, p1 = YN * c - X0 * s;
+      const double p2 = Y0 * c - XN * s, p3 = YN * c - XN * s;
+      const double tmin = fmin(fmin(p0, p1), fmin(p2, p3));
+      const double tmax = fmax(fmax(p0, p1), fmax(p2, p3));
+      const int lo = max((int)floor(tmin * g.ihp - g.hp0) - 2, 0);
+      const int hi = min((int)floor(tmax * g.ihp - g.hp0) + 2, nh - 1);
+      s_lo[tid] = lo;
+      s_cnt[tid] = max(0, min(hi - lo + 1, hmax));
+    }
+    __syncthreads();
+    const int total = nblk * hmax * Q4;
+    for (int e = tid; e < total; e += 256) {
+      const int q4 = e % Q4, r = e / Q4;
+      const int hh = r % hmax, t = r / hmax;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (hh < s_cnt[t]) {
+        const float *p = sino + ((size_t)(a0 + t) * nh + s_lo[t] + hh) * nv + k0 + 4 * q4;
+        if (4 * q4 + 3 < nrow) {
+          __builtin_memcpy(&v, p, sizeof(v));
+        } else {
+          if (4 * q4 < nrow) v.x = p[0];
+          if (4 * q4 + 1 < nrow) v.y = p[1];
+          if (4 * q4 + 2 < nrow) v.z = p[2];
+        }
+      }
+      *reinterpret_cast<float4 *>(s_rows + ((size_t)t * hmax + hh) * R + 4 * q4) = v;
+    }
+    __syncthreads();
+    if (active) {
+      for (int t = 0; t < nblk; t++) {
+        const int a = a0 + t;
+        const double *tab = g.atab + 5 * a;
+        const ParFoot f = par_footprint(g, tab, x0, xn, y0, yn);
+        const int slo = s_lo[t], scnt = s_cnt[t];
+        for (int h = f.lo; h <= f.hi; h++) {
+          double w;
+          const int use = par_weight(g, tab, f, g.h_pix[h], w);
+          if (use == 0) continue;
+          if (use < 0) break;
+          const float wf = (float)w;
+          const int hh = h - slo;
+          if (hh >= 0 && hh < scnt) {
+            const float *pr = s_rows + ((size_t)t * hmax + hh) * R;
+#pragma unroll
+            for (int q4 = 0; q4 < Q4; q4++) {
+              const float4 v = *reinterpret_cast<const float4 *>(pr + 4 * q4);
+              acc[4 * q4] = acc[4 * q4] + (Acc)(v.x * wf);
+              acc[4 * q4 + 1] = acc[4 * q4 + 1] + (Acc)(v.y * wf);
+              acc[4 * q4 + 2] = acc[4 * q4 + 2] + (Acc)(v.z * wf);
+              acc[4 * q4 + 3] = acc[4 * q4 + 3] + (Acc)(v.w * wf);
+            }
+          } else {
+            const float *pr = sino + ((size_t)a * nh + h) * nv + k0;
+#pragma unroll
+            for (int q = 0; q < R; q++)
+              if (q < nrow) acc[q] = acc[q] + (Acc)(pr[q] * wf);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!active) return 0.0;
+  double ss = 0.0;
+#pragma unroll
+  for (int q = 0; q < R; q++)
+    if (q < nk) {
+      const float o = (float)acc[q];
+      vcol[q] = o;
+      ss += (double)o * (double)o;
+    }
+  return ss;
+}
+
 }  // namespace ccpi

@@ -49,126 +49,21 @@ __global__ __launch_bounds__(256) void par_bp_kernel(ParArgs g,
   par_bp_thread<kRows, Acc>(g, blockIdx.x, blockIdx.y, threadIdx.x, sino, vol, true);
 }
 
-// Tiled BP (one row per slice): a workgroup owns a 16 x 16 tile of voxel
-// columns, one column per lane and R consecutive slices per lane.  Angles go
-// in blocks of `ab`: for each angle of the block, the detector columns that
-// the tile's square can reach (its corners' span plus a margin of 2) are
-// staged in LDS with the R rows of the lane's slices (hmax columns at most),
-// one coalesced copy per tile instead of each lane re-reading its columns'
-// rows; every lane then walks its own column's footprint (the reference's
-// FP64 arithmetic, par_footprint / par_weight) and reads the rows from LDS.
-// A footprint column outside the staged span (never expected; rounding
-// guard) is read from global memory.  Angles are visited in order, so with
-// one angle group the result is bit-identical to par_bp_kernel; with G
-// groups (blockIdx.z) each group writes its partial sums of its angle range
-// to part[group] and par_bp_sum_kernel adds them in group order.
+// The tiled BP as one launch: blockIdx.x = tile, y = slice chunk, z = angle
+// group; one group adds into vol, several write their partial sums to
+// part[group] for par_bp_sum_kernel.
 template <int R, class Acc>
 __global__ __launch_bounds__(256) void par_bp_lds_kernel(ParArgs g, int hmax, int ab, int groups,
                                                          const float *__restrict__ sino,
                                                          float *__restrict__ vol,
                                                          float *__restrict__ part) {
-  static_assert(R % 4 == 0, "rows are staged 4 at a time");
   extern __shared__ float smem[];
-  float *s_rows = smem;                                 // [ab][hmax][R]
-  int *s_lo = reinterpret_cast<int *>(smem + (size_t)ab * hmax * R);
-  int *s_cnt = s_lo + ab;
-  const int tid = threadIdx.x;
-  const int ntj = (g.ny + 15) >> 4;
-  const int ti0 = (int)(blockIdx.x / ntj) * 16, tj0 = (int)(blockIdx.x % ntj) * 16;
-  const int i = ti0 + (tid >> 4), j = tj0 + (tid & 15);
-  const bool active = i < g.nx && j < g.ny;
-  const int k0 = blockIdx.y * R;
-  const int nz = g.nz, nv = g.nv, nh = g.nh;
-  const int nk = min(R, nz - k0);
   const int grp = blockIdx.z;
   const int a_begin = (int)((long long)g.na * grp / groups);
   const int a_end = (int)((long long)g.na * (grp + 1) / groups);
-  const size_t colid = (size_t)(active ? i : 0) * g.ny + (active ? j : 0);
-  Acc acc[R];
-  float *__restrict__ vcol = vol + colid * nz + k0;
-#pragma unroll
-  for (int q = 0; q < R; q++)
-    acc[q] = (groups == 1 && active && q < nk) ? (Acc)vcol[q] : (Acc)0;
-  const double x0 = g.ox + (double)i * g.dx, xn = g.ox + (double)(i + 1) * g.dx;
-  const double y0 = g.oy + (double)j * g.dy, yn = g.oy + (double)(j + 1) * g.dy;
-  // tile square
-  const double X0 = g.ox + (double)ti0 * g.dx, XN = g.ox + (double)min(ti0 + 16, g.nx) * g.dx;
-  const double Y0 = g.oy + (double)tj0 * g.dy, YN = g.oy + (double)min(tj0 + 16, g.ny) * g.dy;
-  // rows of the lane's slices that exist (one row per slice)
-  const int nrow = max(0, min(R, min(nv, nz) - k0));
-  constexpr int Q4 = R / 4;
-  for (int a0 = a_begin; a0 < a_end; a0 += ab) {
-    const int nblk = min(ab, a_end - a0);
-    if (tid < nblk) {
-      const double *t = g.atab + 5 * (a0 + tid);
-      const double c = t[0], s = t[1];
-      const double p0 = Y0 * c - X0 * s, p1 = YN * c - X0 * s;
-      const double p2 = Y0 * c - XN * s, p3 = YN * c - XN * s;
-      const double tmin = fmin(fmin(p0, p1), fmin(p2, p3));
-      const double tmax = fmax(fmax(p0, p1), fmax(p2, p3));
-      const int lo = max((int)floor(tmin * g.ihp - g.hp0) - 2, 0);
-      const int hi = min((int)floor(tmax * g.ihp - g.hp0) + 2, nh - 1);
-      s_lo[tid] = lo;
-      s_cnt[tid] = max(0, min(hi - lo + 1, hmax));
-    }
-    __syncthreads();
-    const int total = nblk * hmax * Q4;
-    for (int e = tid; e < total; e += 256) {
-      const int q4 = e % Q4, r = e / Q4;
-      const int hh = r % hmax, t = r / hmax;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (hh < s_cnt[t]) {
-        const float *p = sino + ((size_t)(a0 + t) * nh + s_lo[t] + hh) * nv + k0 + 4 * q4;
-        if (4 * q4 + 3 < nrow) {
-          __builtin_memcpy(&v, p, sizeof(v));
-        } else {
-          if (4 * q4 < nrow) v.x = p[0];
-          if (4 * q4 + 1 < nrow) v.y = p[1];
-          if (4 * q4 + 2 < nrow) v.z = p[2];
-        }
-      }
-      *reinterpret_cast<float4 *>(s_rows + ((size_t)t * hmax + hh) * R + 4 * q4) = v;
-    }
-    __syncthreads();
-    if (active) {
-      for (int t = 0; t < nblk; t++) {
-        const int a = a0 + t;
-        const double *tab = g.atab + 5 * a;
-        const ParFoot f = par_footprint(g, tab, x0, xn, y0, yn);
-        const int slo = s_lo[t], scnt = s_cnt[t];
-        for (int h = f.lo; h <= f.hi; h++) {
-          double w;
-          const int use = par_weight(g, tab, f, g.h_pix[h], w);
-          if (use == 0) continue;
-          if (use < 0) break;
-          const float wf = (float)w;
-          const int hh = h - slo;
-          if (hh >= 0 && hh < scnt) {
-            const float *pr = s_rows + ((size_t)t * hmax + hh) * R;
-#pragma unroll
-            for (int q4 = 0; q4 < Q4; q4++) {
-              const float4 v = *reinterpret_cast<const float4 *>(pr + 4 * q4);
-              acc[4 * q4] = acc[4 * q4] + (Acc)(v.x * wf);
-              acc[4 * q4 + 1] = acc[4 * q4 + 1] + (Acc)(v.y * wf);
-              acc[4 * q4 + 2] = acc[4 * q4 + 2] + (Acc)(v.z * wf);
-              acc[4 * q4 + 3] = acc[4 * q4 + 3] + (Acc)(v.w * wf);
-            }
-          } else {
-            const float *pr = sino + ((size_t)a * nh + h) * nv + k0;
-#pragma unroll
-            for (int q = 0; q < R; q++)
-              if (q < nrow) acc[q] = acc[q] + (Acc)(pr[q] * wf);
-          }
-        }
-      }
-    }
-    __syncthreads();
-  }
-  if (!active) return;
-  float *__restrict__ dst = groups == 1 ? vcol : part + (size_t)grp * g.nx * g.ny * nz + colid * nz + k0;
-#pragma unroll
-  for (int q = 0; q < R; q++)
-    if (q < nk) dst[q] = (float)acc[q];
+  float *dst = groups == 1 ? vol : part + (size_t)grp * g.nx * g.ny * g.nz;
+  (void)par_bp_tile<R, Acc>(g, blockIdx.x, blockIdx.y, a_begin, a_end, hmax, ab, groups == 1,
+                            sino, dst, smem);
 }
 
 // vol += sum of the angle groups' partials, in group order (in Acc).

@@ -66,8 +66,15 @@ def test_batched_64_matches_one_at_a_time(batch64):
         xp, _, npl = cgls(g, b, 20)
         dev = float((xb[i] - xp).abs().max() / xp.abs().max())
         worst = max(worst, dev)
+        # ||A^T r|| falls by orders of magnitude over the run, towards the
+        # level of its float rounding: compared on the scale of its value
+        # after the first iteration
+        nb_i, np_i = np.asarray(nb[i]), np.asarray(npl)
+        ndev = float(np.max(np.abs(nb_i - np_i)) / np_i[0])
+        print(f"problem {i} {g.shape}: iterate {dev:.2e}, norms {ndev:.2e} of ||A^T b||, "
+              f"last norm {np_i[-1] / np_i[0]:.2e} of it", flush=True)
         assert dev <= 1e-4, (i, g.shape, dev)
-        assert np.allclose(nb[i], npl, rtol=1e-4)
+        assert ndev <= 1e-4, (i, g.shape, ndev)
     print(f"max deviation batched vs one at a time {worst:.2e}", flush=True)
 
 
