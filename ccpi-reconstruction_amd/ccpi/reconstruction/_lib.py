@@ -23,7 +23,7 @@ CCPI_REG_TV = 1
 
 # every symbol include/ccpi_hip.h declares
 EXPORTS = (
-    "ccpi_pb_f2D", "ccpi_pb_b2D", "ccpi_cb_f2D", "ccpi_cb_b2D",
+    "ccpi_pb_f2D", "ccpi_pb_f2D_noalloc", "ccpi_pb_b2D", "ccpi_cb_f2D", "ccpi_cb_b2D",
     "ccpi_forward_project", "ccpi_backward_project",
     "ccpi_cgls", "ccpi_sirt", "ccpi_mlem", "ccpi_regularize", "ccpi_cgls_reg",
     "ccpi_xtek_find_centre",
@@ -72,6 +72,7 @@ def lib():
     dp, fp, vp, i = C.POINTER(C.c_double), C.c_void_p, C.c_void_p, C.c_int
     d = C.c_double
     L.ccpi_pb_f2D.argtypes = [dp, dp, dp, i, i, i, dp, dp, i, i, i, fp, fp, vp]
+    L.ccpi_pb_f2D_noalloc.argtypes = [dp, dp, dp, i, i, i, dp, dp, i, i, i, fp, fp, vp]
     L.ccpi_pb_b2D.argtypes = [dp, dp, dp, fp, fp, i, i, i, dp, dp, i, i, i, vp]
     L.ccpi_cb_f2D.argtypes = [d, d, d, d, dp, dp, dp, fp, fp, i, i, i, dp, dp, i, i, i, vp]
     L.ccpi_cb_b2D.argtypes = [d, d, d, d, dp, dp, dp, fp, fp, i, i, i, dp, dp, i, i, i, vp]
@@ -94,7 +95,7 @@ def lib():
     L.ccpi_cgls_destroy.argtypes = [vp]
     L.ccpi_projector_nnz.argtypes = [gp, C.POINTER(C.c_ulonglong), vp]
     L.ccpi_cone_bp_layout.argtypes = [gp, ip, ip, ip, ip]
-    for name in ("ccpi_pb_f2D", "ccpi_pb_b2D", "ccpi_cb_f2D", "ccpi_cb_b2D",
+    for name in ("ccpi_pb_f2D", "ccpi_pb_f2D_noalloc", "ccpi_pb_b2D", "ccpi_cb_f2D", "ccpi_cb_b2D",
                  "ccpi_forward_project", "ccpi_backward_project", "ccpi_cgls",
                  "ccpi_sirt", "ccpi_mlem", "ccpi_regularize", "ccpi_cgls_reg",
                  "ccpi_xtek_find_centre", "ccpi_pb_cgls_batched", "ccpi_cgls_create",

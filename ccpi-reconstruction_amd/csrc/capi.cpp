@@ -51,6 +51,18 @@ int ccpi_pb_f2D(const double *h_pixels, const double *v_pixels, const double *an
   });
 }
 
+int ccpi_pb_f2D_noalloc(const double *h_pixels, const double *v_pixels, const double *angles,
+                        int n_angles, int n_h, int n_v, const double vox_origin[3],
+                        const double vox_size[3], int nx, int ny, int nz, float *pixels,
+                        const float *voxels, void *stream) {
+  return guarded([&] {
+    CCPI_REQUIRE(pixels && voxels, CCPI_E_ARG, "null data pointer");
+    const ccpi::Geometry G = ccpi::Geometry::parallel(h_pixels, v_pixels, angles, n_angles,
+                                                      n_h, n_v, vox_origin, vox_size, nx, ny, nz);
+    ccpi::par_fp_noalloc(G, voxels, pixels, S(stream));
+  });
+}
+
 int ccpi_pb_b2D(const double *h_pixels, const double *v_pixels, const double *angles,
                 const float *pixels, float *voxels, int n_angles, int n_h, int n_v,
                 const double vox_origin[3], const double vox_size[3], int nx, int ny, int nz,

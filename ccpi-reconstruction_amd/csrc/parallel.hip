@@ -22,6 +22,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -313,6 +316,82 @@ void par_bp(const ParPlan &P, const float *sino, float *vol, float *scratch, hip
     par_bp_launch<double>(P, sino, vol, scratch, st);
   else
     par_bp_launch<float>(P, sino, vol, scratch, st);
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free forward projection
+// ---------------------------------------------------------------------------
+
+// The per-call tables of par_fp_noalloc live in this file-scope device
+// array instead of a cached plan: [atab | h_pix | map | rlo | rhi], each
+// part 256-byte aligned.  Every device the library runs on holds one copy
+// until the process ends; calls on one device share it, so calls that
+// overlap on one device (different streams, host threads) race.  In host
+// code `scratch` names a host-side stand-in: kernels get its device address
+// from hipGetSymbolAddress (scratch_address below), never the name.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, unsigned char *> g_scratch_ptr;  // device -> address of `scratch`
+
+// Device address of the current device's `scratch`, looked up once per
+// device (with that device current).
+unsigned char *scratch_address() {
+  int dev = 0;
+  CCPI_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) return it->second;
+  void *p = nullptr;
+  CCPI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  CCPI_REQUIRE(p != nullptr, CCPI_E_HIP, "no device address for the projector scratch");
+  g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+  return static_cast<unsigned char *>(p);
+}
+
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+}  // namespace
+
+size_t par_noalloc_bytes(const Geometry &G) {
+  return align256((size_t)G.na * 5 * sizeof(double)) + align256((size_t)G.nh * sizeof(double)) +
+         align256((size_t)G.nv * sizeof(int)) + 2 * align256((size_t)G.nz * sizeof(int));
+}
+
+void par_fp_noalloc(const Geometry &G, const float *vol, float *sino, hipStream_t st) {
+  const size_t need = par_noalloc_bytes(G);
+  CCPI_REQUIRE(need <= kNoAllocScratchBytes, CCPI_E_LIMIT,
+               "allocation-free parallel FP: the geometry's tables need " +
+                   std::to_string(need) + " bytes, more than its " +
+                   std::to_string(kNoAllocScratchBytes) + "-byte scratch");
+  ParArgs A;
+  const ParTables T = par_tables(G, A);
+  // one host image of the tables, copied into the scratch in one transfer
+  std::vector<unsigned char> img(need, 0);
+  size_t off = 0;
+  auto place = [&](const void *src, size_t bytes) {
+    const size_t at = off;
+    if (bytes) std::memcpy(img.data() + at, src, bytes);
+    off += align256(bytes);
+    return at;
+  };
+  const size_t o_atab = place(T.atab.data(), T.atab.size() * sizeof(double));
+  const size_t o_hpix = place(G.h_pix.data(), G.h_pix.size() * sizeof(double));
+  const size_t o_map = place(T.map.data(), T.map.size() * sizeof(int));
+  const size_t o_rlo = place(T.rlo.data(), T.rlo.size() * sizeof(int));
+  const size_t o_rhi = place(T.rhi.data(), T.rhi.size() * sizeof(int));
+  unsigned char *base = scratch_address();
+  // stream-ordered: an earlier call's kernel on this stream has read the
+  // previous tables before these land
+  CCPI_HIP(hipMemcpyAsync(base, img.data(), need, hipMemcpyHostToDevice, st));
+  A.atab = reinterpret_cast<const double *>(base + o_atab);
+  A.h_pix = reinterpret_cast<const double *>(base + o_hpix);
+  A.map = reinterpret_cast<const int *>(base + o_map);
+  A.rlo = reinterpret_cast<const int *>(base + o_rlo);
+  A.rhi = reinterpret_cast<const int *>(base + o_rhi);
+  ParPlan P;
+  P.args = A;
+  par_fp(P, vol, sino, st);
 }
 
 }  // namespace ccpi

@@ -161,6 +161,11 @@ void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, hipStream_
                     bool f64 = false);
 
 void par_build_plan(ParPlan &P, hipStream_t st);
+// Parallel FP without any device allocation: the geometry's tables go into
+// a fixed file-scope device array of kNoAllocScratchBytes (parallel.hip).
+constexpr size_t kNoAllocScratchBytes = size_t(1) << 20;
+size_t par_noalloc_bytes(const Geometry &g);
+void par_fp_noalloc(const Geometry &g, const float *vol, float *sino, hipStream_t st);
 void par_fp(const ParPlan &P, const float *vol, float *sino, hipStream_t st, bool f64 = false);
 // scratch: par_bp_scratch_floats(P) floats (partials of the angle groups),
 // unused when that is 0

@@ -62,6 +62,22 @@ int ccpi_pb_f2D(const double *h_pixels, const double *v_pixels,
                 int nx, int ny, int nz, float *pixels, const float *voxels,
                 void *stream);
 
+/* ccpi_pb_f2D that allocates no device memory: no counterpart in the
+   reference.  For callers that cannot allocate (code under HIP graph
+   capture, serving loops, harnesses that forbid hipMalloc).  Same
+   arguments and result as ccpi_pb_f2D; instead of a cached plan the
+   geometry's tables go into a fixed 1 MiB device array of the library
+   (per GPU, held until the process ends), so the geometry is limited to
+   40 n_angles + 8 n_h + 4 n_v + 8 nz bytes (each part rounded up to 256)
+   <= 1 MiB; larger ones return CCPI_E_LIMIT before any device work.
+   Calls on one GPU share that array: calls that overlap on one GPU (other
+   streams, other host threads) race; calls on one stream are ordered. */
+int ccpi_pb_f2D_noalloc(const double *h_pixels, const double *v_pixels,
+                        const double *angles, int n_angles, int n_h, int n_v,
+                        const double vox_origin[3], const double vox_size[3],
+                        int nx, int ny, int nz, float *pixels,
+                        const float *voxels, void *stream);
+
 /* x += A_par^T y.  Replaces CCPi::parallel_beam::b2D
    (Core/src/instruments.hpp:272-277, Core/src/p2D.cpp:950-1087). */
 int ccpi_pb_b2D(const double *h_pixels, const double *v_pixels,

@@ -131,3 +131,20 @@ def test_find_centre_rejects_bad_arguments():
     assert L.ccpi_xtek_find_centre(C.byref(g), dummy, 5, C.byref(y), None) == -1
     assert b"v_slice" in L.ccpi_last_error()
 
+
+def test_noalloc_fp_rejects_tables_beyond_its_scratch():
+    """The allocation-free parallel FP holds a geometry's tables in a fixed
+    1 MiB device array: 40 bytes per angle alone exceed it at 30000 angles,
+    which it rejects before any device work."""
+    L = _lib.lib()
+    na = 30000
+    h = np.linspace(-4, 4, 9)
+    v = np.array([0.5])
+    ang = np.linspace(0, np.pi, na, endpoint=False)
+    o = np.array([-4.0, -4.0, 0.0])
+    s = np.array([1.0, 1.0, 1.0])
+    dummy = C.c_void_p(16)  # never dereferenced: the size check fails first
+    st = L.ccpi_pb_f2D_noalloc(_dp(h), _dp(v), _dp(ang), na, len(h), 1, _dp(o), _dp(s), 8, 8, 1,
+                               dummy, dummy, None)
+    assert st == -5
+    assert b"scratch" in L.ccpi_last_error()
