@@ -49,6 +49,7 @@ struct BatchProb {
   int fp_lr;           // FP lanes per ray
   int bp_tiled;        // BP items are 16 x 16 column tiles staged in LDS
   int bp_hmax, bp_ab;  // their staged detector columns per angle, angles per block
+  int bp_groups;       // angle groups of the problem's one-problem tiled BP
 };
 
 struct BatchLists {
@@ -80,12 +81,14 @@ __device__ inline double wave_sum_d(double v) {
 
 // FP of d into Ad for every problem (mode 0), or BP of r into s (mode 1),
 // or BP of r into d (mode 2, initial direction).  FP work is taken per
-// wavefront: wave item w is quarter w % 4 (64 threads) of block item w / 4,
-// so no wave waits at a barrier for the longest walk of its workgroup.  BP
-// work is taken per workgroup: a 16 x 16 column tile whose detector rows
-// the workgroup stages in LDS (par_bp_tile, the one-problem path's tiled
-// body) or, for problems the tiled form does not take, 256 columns of the
-// column-per-lane body.  Every wave of an item leaves one FP64 partial norm.
+// wavefront: a wave takes a block item (256 rays' lanes) and runs its four
+// 64-lane quarters in turn, so no wave waits at a barrier for the longest
+// walk of its workgroup.  BP work is taken per workgroup: a 16 x 16 column
+// tile whose detector rows the workgroup stages in LDS (par_bp_tile, the
+// one-problem path's tiled body, with its angle groups folded in registers
+// so the sums are those of the one-problem launch) or, for problems the
+// tiled form does not take, 256 columns of the column-per-lane body.  Every
+// quarter (FP) or wave (BP) of an item leaves one FP64 partial norm.
 template <int RC, int MODE>
 __global__ __launch_bounds__(256) void batch_proj_kernel(BatchProb *__restrict__ P,
                                                          BatchLists L) {
@@ -94,21 +97,24 @@ __global__ __launch_bounds__(256) void batch_proj_kernel(BatchProb *__restrict__
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (MODE == 0) {
-    const int total = 4 * off[L.count];
+    // a wavefront takes the four 64-thread quarters of a block item one
+    // after another: one problem lookup (a binary search and the
+    // descriptor's scalar loads) per 256 threads' worth of rays
+    const int total = off[L.count];
     const int wid = blockIdx.x * 4 + wave;
-    for (int w = wid; w < total; w += 4 * gridDim.x) {
-      const int item = w >> 2;
-      const int tid = (w & 3) * 64 + lane;
+    for (int item = wid; item < total; item += 4 * gridDim.x) {
       const int p = find_prob(off, L.count, item);
       const BatchProb &B = P[p];
       // the problem's geometry as a local value (scalar registers), as the
       // one-problem kernels get it through their arguments
       const ParArgs g = B.g;
       const int local = item - off[p];
-      double ss =
-          par_fp_lanes(g, local % B.fp_nbx, local / B.fp_nbx, tid, B.fp_lr, B.d, B.Ad, false);
-      ss = wave_sum_d(ss);
-      if (lane == 0) part[w] = ss;
+      for (int qtr = 0; qtr < 4; ++qtr) {
+        double ss = par_fp_lanes(g, local % B.fp_nbx, local / B.fp_nbx, qtr * 64 + lane,
+                                 B.fp_lr, B.d, B.Ad, false);
+        ss = wave_sum_d(ss);
+        if (lane == 0) part[4 * item + qtr] = ss;
+      }
     }
     return;
   }
@@ -122,8 +128,9 @@ __global__ __launch_bounds__(256) void batch_proj_kernel(BatchProb *__restrict__
     float *dst = MODE == 1 ? B.s : B.d;
     double ss;
     if (B.bp_tiled) {
-      ss = par_bp_tile<RC, float>(g, local % B.bp_nbx, local / B.bp_nbx, 0, g.na, B.bp_hmax,
-                                  B.bp_ab, false, B.r, dst, smem);
+      ss = par_bp_tile<RC, float, true>(g, local % B.bp_nbx, local / B.bp_nbx, 0, g.na,
+                                        B.bp_hmax, B.bp_ab, false, B.r, dst, smem,
+                                        B.bp_groups);
     } else {
       ss = par_bp_thread<RC>(g, local % B.bp_nbx, local / B.bp_nbx, threadIdx.x, B.r, dst,
                              false);
@@ -259,8 +266,8 @@ struct BatchRun {
 };
 
 // Item lists for row/slice blocks of `rc`.  A problem whose plan takes the
-// tiled BP gets tile items of rc slices, its staged span from the plan and
-// as many angles per staged block as fit 32 KB.
+// tiled BP gets tile items of rc slices, its staged span and angle groups
+// from the plan and as many angles per staged block as fit 32 KB.
 void layout(BatchRun &R, int rc, hipStream_t st) {
   std::vector<int> off(4 * (R.count + 1), 0);
   R.bp_lds = 0;
@@ -275,6 +282,7 @@ void layout(BatchRun &R, int rc, hipStream_t st) {
     if (S.tiled) {
       const size_t per_angle = sizeof(float) * (size_t)S.hmax * rc + 2 * sizeof(int);
       B.bp_hmax = S.hmax;
+      B.bp_groups = S.groups;
       B.bp_ab = (int)std::max<size_t>(1, std::min<size_t>(8, (32 * 1024) / per_angle));
       R.bp_lds = std::max(R.bp_lds, per_angle * B.bp_ab);
       B.bp_nbx = ((g.nx + 15) / 16) * ((g.ny + 15) / 16);

@@ -560,7 +560,6 @@ __device__ inline void live_rows(const ConeArgs &g, float a0, int &vlo, int &vhi
 }
 
 __device__ inline int lds_pad4(int n) { return (n + 3) & ~3; }
-inline int lds_pad4_host(int n) { return (n + 3) & ~3; }
 
 // Column extent in world units.
 struct ColBox {
@@ -1074,92 +1073,6 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
   }
 }
 
-// Phased BP, lockstep form: a workgroup of kLockWaves wavefronts owns a
-// 4 x 4 block of voxel columns (one per wave; four blocks make an 8 x 8
-// tile, the tiles of one XCD consecutive), and the waves sweep their
-// columns' records kLockAngles angles at a time, meeting at a barrier after
-// each angle block.  Neighbouring columns need mostly the same detector
-// rows at the same angles, so while the 16 waves are inside one angle block
-// the rows they gather come from L2 once the first wave has fetched them
-// (unsynchronised waves drift apart by more angles than L2 holds rows for).
-// The per-record work is the phased form's (bp_phased_block); within an
-// angle block the records keep their order, so with one row block and one
-// row set per column the sums are the unsynchronised kernel's.
-constexpr int kLockWaves = 16;
-constexpr int kLockAngles = 2;
-
-template <int S, class AccT>
-__global__ __launch_bounds__(64 * kLockWaves) void cone_bp_lock_kernel(
-    ConeArgs g, const BpCol *__restrict__ cols, const BpRec *__restrict__ recs,
-    const float *__restrict__ dpix, float *__restrict__ vol) {
-  extern __shared__ float smem[];
-  const int nz = g.nz, nv = g.nv;
-  float *s_vzx = smem;
-  for (int q = threadIdx.x; q <= nz + 1; q += 64 * kLockWaves) s_vzx[q] = g.vz[max(q - 1, 0)];
-  __syncthreads();
-  const float *s_vz1 = s_vzx + 1;
-  const int astride = lds_pad4(nz + 2);
-  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int lane = threadIdx.x & 63;
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int tile = lb >> 2, quad = lb & 3;
-  const int ntj = (g.ny + 7) >> 3;
-  const int ci = (tile / ntj) * 8 + (quad >> 1) * 4 + (wave >> 2);
-  const int cj = (tile % ntj) * 8 + (quad & 1) * 4 + (wave & 3);
-  const bool incol = ci < g.nx && cj < g.ny;
-  const int colid = incol ? ci * g.ny + cj : 0;
-  BpCol C{};
-  if (incol) C = cols[colid];
-  // every wave runs the angle loop (its barriers), working only when its
-  // column exists and some ray crosses it
-  const bool active = incol && C.n > 0;
-  AccT *acc = reinterpret_cast<AccT *>(smem + astride) + wave * astride;
-  float *__restrict__ vcol = vol + (size_t)colid * nz;
-  if (active) {
-    for (int q = lane; q < astride; q += 64) acc[q] = (AccT)0;
-    __builtin_amdgcn_wave_barrier();
-    for (int k = lane; k < nz; k += 64) acc[k + 1] = vcol[k];
-    __builtin_amdgcn_wave_barrier();
-  }
-  const int min_v = C.min_v, max_v = C.max_v;
-  const bool one_set = (min_v <= g.mid) && (max_v >= g.mid);
-  const int nph = g.bp_phases;
-  const int ph = lane % nph;
-  constexpr int span = 64 * S;
-  float inv_dz;
-  asm("v_mov_b32 %0, %1" : "=v"(inv_dz) : "s"(g.inv_dz));
-  const BpRec *__restrict__ R = recs + C.off;
-  int e = 0;
-  for (int a0 = 0; a0 < g.na; a0 += kLockAngles) {
-    const uint32_t lim = (uint32_t)min(a0 + kLockAngles, g.na) * (uint32_t)g.nh;
-    int e1 = e;
-    if (active) {
-      // records are in angle-then-h order: those of this angle block are a
-      // prefix of the rest, counted 64 at a time
-      for (;;) {
-        const int q = e1 + lane;
-        const bool in = q < C.n && R[q].x < lim;
-        const int cnt = __popcll(__ballot(in));
-        e1 += cnt;
-        if (cnt < 64) break;
-      }
-      if (e1 > e)
-        for (int vb = 0; vb < nv; vb += span)
-          bp_phased_block<S, AccT>(
-              g, R + e, e1 - e, dpix, s_vz1, acc + 1, vb, lane, nph, ph, inv_dz,
-              one_set && g.bp_main_safe && vb >= min_v && min(vb + span, nv) <= max_v, one_set,
-              min_v, max_v);
-    }
-    e = e1;
-    __syncthreads();
-  }
-  if (active) {
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    for (int k = lane; k < nz; k += 64) vcol[k] = (float)acc[k + 1];
-  }
-}
-
 // dpix[r][v] = pix[r][v] * d_conv[h][v] for v < nv, 0 up to the padded row
 // stride nvp (c2D.cpp:1105-1114).
 __global__ void cone_scale_kernel(const float *__restrict__ pix,
@@ -1503,25 +1416,6 @@ static void cone_bp_launch(const ConePlan &P, const float *dpix, float *vol, hip
                "cone BP: nz too large for LDS with FP64 accumulation");
   const BpCol *bc = P.bpcol.as<BpCol>();
   const BpRec *br = P.bprec.as<BpRec>();
-  if (A.bp_phased) {
-    // lockstep form: 4 x 4 column blocks of kLockWaves wavefronts
-    const size_t pad = (size_t)lds_pad4_host(A.nz + 2);
-    const size_t llds = sizeof(float) * pad + sizeof(AccT) * kLockWaves * pad;
-    CCPI_REQUIRE(llds <= 160 * 1024, CCPI_E_LIMIT, "cone BP: nz too large for LDS");
-    const dim3 lgrid((unsigned)(((A.nx + 7) / 8) * ((A.ny + 7) / 8) * 4)),
-        lblock(64 * kLockWaves);
-    if (A.bp_stride == 4)
-      hipLaunchKernelGGL((cone_bp_lock_kernel<4, AccT>), lgrid, lblock, llds, st, A, bc, br,
-                         dpix, vol);
-    else if (A.bp_stride == 8)
-      hipLaunchKernelGGL((cone_bp_lock_kernel<8, AccT>), lgrid, lblock, llds, st, A, bc, br,
-                         dpix, vol);
-    else
-      hipLaunchKernelGGL((cone_bp_lock_kernel<16, AccT>), lgrid, lblock, llds, st, A, bc, br,
-                         dpix, vol);
-    CCPI_HIP(hipGetLastError());
-    return;
-  }
   const dim3 grid(blocks), block(64 * kBpWaves);
   if (A.bp_phased && A.bp_stride == 4)
     hipLaunchKernelGGL((cone_bp_kernel<4, true, AccT>), grid, block, lds, st, A, bc, br, dpix,

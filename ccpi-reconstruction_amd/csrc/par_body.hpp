@@ -94,6 +94,117 @@ __device__ inline void par_walk(const ParArgs &g, int a, int h, Add &&add) {
   }
 }
 
+// The walk of par_walk, G cells at a time: emit(cx, cy, l, n) receives the
+// next n <= G cells (n < G only for the last group) in the walk's order,
+// so a caller can issue the group's gathers before it accumulates them.
+template <int G, class Emit>
+__device__ inline void par_walk_groups(const ParArgs &g, int a, int h, Emit &&emit) {
+  const int nx = g.nx, ny = g.ny;
+  const double c = g.atab[5 * a], s = g.atab[5 * a + 1];
+  const double p2x = c * g.D - s * g.h_pix[h];
+  const double p2y = s * g.D + c * g.h_pix[h];
+  int cx[G], cy[G];
+  float l[G];
+  if (fabs(c) < kEps) {
+    if (fabs(s) >= kEps) {  // line parallel to y (p2D.cpp:233-260)
+      const int i = (int)floor((p2x - g.ox) / g.dx);
+      if (i >= 0 && i < nx) {
+        for (int j0 = 0; j0 < ny; j0 += G) {
+#pragma unroll
+          for (int q = 0; q < G; q++) {
+            cx[q] = i;
+            cy[q] = s < 0.0 ? ny - 1 - (j0 + q) : j0 + q;
+            l[q] = (float)g.dy;
+          }
+          emit(cx, cy, l, min(G, ny - j0));
+        }
+      }
+    }
+  } else if (fabs(s) < kEps) {  // line parallel to x (p2D.cpp:261-283)
+    const int j = (int)floor((p2y - g.oy) / g.dy);
+    if (j >= 0 && j < ny) {
+      for (int i0 = 0; i0 < nx; i0 += G) {
+#pragma unroll
+        for (int q = 0; q < G; q++) {
+          cx[q] = c < 0.0 ? nx - 1 - (i0 + q) : i0 + q;
+          cy[q] = j;
+          l[q] = (float)g.dx;
+        }
+        emit(cx, cy, l, min(G, nx - i0));
+      }
+    }
+  } else {  // general line (p2D.cpp:284-458), steps as par_walk
+    const double ix = 1.0 / c, iy = 1.0 / s;
+    const double ax0 = (g.ox - p2x) * ix, ay0 = (g.oy - p2y) * iy;
+    const double axn = (g.ox + (double)nx * g.dx - p2x) * ix;
+    const double ayn = (g.oy + (double)ny * g.dy - p2y) * iy;
+    const double amin = fmax(fmax(fmin(ax0, axn), fmin(ay0, ayn)), -g.dconv);
+    const double amax = fmin(fmin(fmax(ax0, axn), fmax(ay0, ayn)), 0.0);
+    if (amin < amax - kEps) {
+      const int sx = c > 0.0 ? 1 : -1, sy = s > 0.0 ? 1 : -1;
+      int x, y;
+      {
+        const double xface = sx > 0 ? ax0 : axn, yface = sy > 0 ? ay0 : ayn;
+        const int xin = sx > 0 ? 0 : nx - 1, yin = sy > 0 ? 0 : ny - 1;
+        bool on_x = amin == xface, on_y = amin == yface;
+        if (sx > 0 && sy < 0) {
+          if (on_y) on_x = false;
+        } else if (!(sx < 0 && sy < 0)) {
+          if (on_x) on_y = false;
+        }
+        if (on_x) {
+          x = xin;
+          y = on_y ? yin : (int)floor((p2y + amin * s - g.oy) / g.dy);
+        } else if (on_y) {
+          y = yin;
+          x = (int)floor((p2x + amin * c - g.ox) / g.dx);
+        } else {
+          x = 0;
+          y = 0;
+        }
+        x = min(max(x, 0), nx - 1);
+        y = min(max(y, 0), ny - 1);
+      }
+      const int bxs = sx > 0, bys = sy > 0;
+      double ap = amin;
+      while (x >= 0 && x < nx && y >= 0 && y < ny) {
+        int n = 0;
+#pragma unroll
+        for (int q = 0; q < G; q++) {
+          const bool in = x >= 0 && x < nx && y >= 0 && y < ny;
+          cx[q] = in ? x : 0;
+          cy[q] = in ? y : 0;
+          l[q] = 0.f;
+          if (in) {
+            const double nxa = (g.ox + (double)(x + bxs) * g.dx - p2x) * ix;
+            const double nya = (g.oy + (double)(y + bys) * g.dy - p2y) * iy;
+            double nap;
+            if (nxa < nya - kEps) {
+              nap = nxa;
+              x += sx;
+            } else if (nya < nxa - kEps) {
+              nap = nya;
+              y += sy;
+            } else {
+              nap = fmax(nxa, nya);
+              x += sx;
+              y += sy;
+            }
+            l[q] = (float)(nap - ap);
+            ap = nap;
+            n = q + 1;
+          }
+        }
+        emit(cx, cy, l, n);
+      }
+    }
+  }
+}
+
+// Cells walked per group in the one-row-per-slice FP: the group's column
+// reads are all in flight before the in-order accumulation
+constexpr int kParFpGroup = 8;
+
 // FP of detector rows [v0, v0 + RC) of ray r (clipped to nv): `acc_in`
 // selects y += A x (reference f2D) or y = A x.  Returns the sum of squares
 // of the rows written.  Acc: float row sums (the reference's), or double
@@ -118,31 +229,54 @@ __device__ inline double par_fp_rows(const ParArgs &g, int r, int v0,
   // one row per slice (map_type 1): the rows are the contiguous slices
   // v0 .. v0 + nval - 1 of each column, read 16 bytes at a time
   const int nval = g.map_type == 1 ? max(0, min(nrow, nz - v0)) : 0;
-  par_walk(g, a, h, [&](int x, int y, float l) {
-    const float *__restrict__ col = vol + ((size_t)x * ny + y) * nz;
-    if (g.map_type == 1) {
-      const float *__restrict__ cv = col + v0;
+  if (g.map_type == 1) {
+    // kParFpGroup cells at a time: all their column reads are issued before
+    // the accumulation, which adds the cells in the walk's order (the same
+    // float operations as one cell at a time)
+    constexpr int Q4 = RC / 4;
+    par_walk_groups<kParFpGroup>(g, a, h, [&](const int *cx, const int *cy, const float *l,
+                                              int n) {
+      float4 f[kParFpGroup][Q4];
 #pragma unroll
-      for (int q4 = 0; q4 < RC / 4; q4++) {
-        if (4 * q4 + 3 < nval) {
-          float4 f;
-          __builtin_memcpy(&f, cv + 4 * q4, sizeof(f));
-          acc[4 * q4] = acc[4 * q4] + (Acc)(f.x * l);
-          acc[4 * q4 + 1] = acc[4 * q4 + 1] + (Acc)(f.y * l);
-          acc[4 * q4 + 2] = acc[4 * q4 + 2] + (Acc)(f.z * l);
-          acc[4 * q4 + 3] = acc[4 * q4 + 3] + (Acc)(f.w * l);
-        } else {
+      for (int q = 0; q < kParFpGroup; q++) {
+        const float *__restrict__ cv = vol + ((size_t)cx[q] * ny + cy[q]) * nz + v0;
 #pragma unroll
-          for (int q = 4 * q4; q < 4 * q4 + 4; q++)
-            if (q < nval) acc[q] = acc[q] + (Acc)(cv[q] * l);
+        for (int q4 = 0; q4 < Q4; q4++) {
+          float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (q < n) {
+            if (4 * q4 + 3 < nval) {
+              __builtin_memcpy(&v, cv + 4 * q4, sizeof(v));
+            } else {
+              if (4 * q4 < nval) v.x = cv[4 * q4];
+              if (4 * q4 + 1 < nval) v.y = cv[4 * q4 + 1];
+              if (4 * q4 + 2 < nval) v.z = cv[4 * q4 + 2];
+            }
+          }
+          f[q][q4] = v;
         }
       }
-      return;
-    }
 #pragma unroll
-    for (int q = 0; q < RC; q++)
-      if (kmap[q] >= 0) acc[q] = acc[q] + (Acc)(col[kmap[q]] * l);
-  });
+      for (int q = 0; q < kParFpGroup; q++) {
+        if (q < n) {
+#pragma unroll
+          for (int q4 = 0; q4 < Q4; q4++) {
+            const float4 v = f[q][q4];
+            if (4 * q4 < nval) acc[4 * q4] = acc[4 * q4] + (Acc)(v.x * l[q]);
+            if (4 * q4 + 1 < nval) acc[4 * q4 + 1] = acc[4 * q4 + 1] + (Acc)(v.y * l[q]);
+            if (4 * q4 + 2 < nval) acc[4 * q4 + 2] = acc[4 * q4 + 2] + (Acc)(v.z * l[q]);
+            if (4 * q4 + 3 < nval) acc[4 * q4 + 3] = acc[4 * q4 + 3] + (Acc)(v.w * l[q]);
+          }
+        }
+      }
+    });
+  } else {
+    par_walk(g, a, h, [&](int x, int y, float l) {
+      const float *__restrict__ col = vol + ((size_t)x * ny + y) * nz;
+#pragma unroll
+      for (int q = 0; q < RC; q++)
+        if (kmap[q] >= 0) acc[q] = acc[q] + (Acc)(col[kmap[q]] * l);
+    });
+  }
   double ss = 0.0;
 #pragma unroll
   for (int q = 0; q < RC; q++)
@@ -424,10 +558,14 @@ __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid
 // a_end).  The result goes to `dst` (a volume-layout array), added to its
 // contents when acc_in.  Returns the thread's sum of squares of the values
 // it wrote (0 for threads outside the volume).  smem: par_bp_lds_bytes.
-template <int R, class Acc>
+// FOLD: the angles are split into `groups` ranges as par_bp_lds_kernel
+// splits them, each summed from zero and the sums added in group order in
+// registers, so that one item gives bit for bit what the grouped launch
+// and par_bp_sum_kernel give (the batched kernel's tiled items).
+template <int R, class Acc, bool FOLD = false>
 __device__ double par_bp_tile(const ParArgs &g, int tile, int zc, int a_begin, int a_end,
                               int hmax, int ab, bool acc_in, const float *__restrict__ sino,
-                              float *__restrict__ dst_vol, float *smem) {
+                              float *__restrict__ dst_vol, float *smem, int groups = 1) {
   static_assert(R % 4 == 0, "rows are staged 4 at a time");
   float *s_rows = smem;                                 // [ab][hmax][R]
   int *s_lo = reinterpret_cast<int *>(smem + (size_t)ab * hmax * R);
@@ -444,7 +582,9 @@ __device__ double par_bp_tile(const ParArgs &g, int tile, int zc, int a_begin, i
   Acc acc[R];
   float *__restrict__ vcol = dst_vol + colid * nz + k0;
 #pragma unroll
-  for (int q = 0; q < R; q++) acc[q] = (acc_in && active && q < nk) ? (Acc)vcol[q] : (Acc)0;
+  for (int q = 0; q < R; q++)
+    acc[q] = (!FOLD && acc_in && active && q < nk) ? (Acc)vcol[q] : (Acc)0;
+  Acc tot[R];  // FOLD: sum of the groups done so far
   const double x0 = g.ox + (double)i * g.dx, xn = g.ox + (double)(i + 1) * g.dx;
   const double y0 = g.oy + (double)j * g.dy, yn = g.oy + (double)(j + 1) * g.dy;
   // tile square
@@ -452,80 +592,99 @@ __device__ double par_bp_tile(const ParArgs &g, int tile, int zc, int a_begin, i
   const double Y0 = g.oy + (double)tj0 * g.dy, YN = g.oy + (double)min(tj0 + 16, g.ny) * g.dy;
   // rows of the lane's slices that exist (one row per slice)
   const int nrow = max(0, min(R, min(nv, nz) - k0));
+  // rows of 16-byte aligned start (nv a multiple of 4) are staged 16 bytes
+  // at a time, others one float at a time
+  const bool aligned = (nv & 3) == 0;
   constexpr int Q4 = R / 4;
-  for (int a0 = a_begin; a0 < a_end; a0 += ab) {
-    const int nblk = min(ab, a_end - a0);
-    if (tid < nblk) {
-      const double *t = g.atab + 5 * (a0 + tid);
-      const double c = t[0], s = t[1];
-      const double p0 = Y0 * c - X0 * s, p1 = YN * c - X0 * s;
-      const double p2 = Y0 * c - XN * s, p3 = YN * c - XN * s;
-      const double tmin = fmin(fmin(p0, p1), fmin(p2, p3));
-      const double tmax = fmax(fmax(p0, p1), fmax(p2, p3));
-      const int lo = max((int)floor(tmin * g.ihp - g.hp0) - 2, 0);
-      const int hi = min((int)floor(tmax * g.ihp - g.hp0) + 2, nh - 1);
-      s_lo[tid] = lo;
-      s_cnt[tid] = max(0, min(hi - lo + 1, hmax));
+  const int ngrp = FOLD ? groups : 1;
+  for (int grp = 0; grp < ngrp; grp++) {
+    const int ga = FOLD ? a_begin + (int)((long long)(a_end - a_begin) * grp / groups) : a_begin;
+    const int gb =
+        FOLD ? a_begin + (int)((long long)(a_end - a_begin) * (grp + 1) / groups) : a_end;
+    if (FOLD) {
+#pragma unroll
+      for (int q = 0; q < R; q++) acc[q] = (Acc)0;
     }
-    __syncthreads();
-    const int total = nblk * hmax * Q4;
-    for (int e = tid; e < total; e += 256) {
-      const int q4 = e % Q4, r = e / Q4;
-      const int hh = r % hmax, t = r / hmax;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (hh < s_cnt[t]) {
-        const float *p = sino + ((size_t)(a0 + t) * nh + s_lo[t] + hh) * nv + k0 + 4 * q4;
-        if (4 * q4 + 3 < nrow) {
-          __builtin_memcpy(&v, p, sizeof(v));
-        } else {
-          if (4 * q4 < nrow) v.x = p[0];
-          if (4 * q4 + 1 < nrow) v.y = p[1];
-          if (4 * q4 + 2 < nrow) v.z = p[2];
-        }
+    for (int a0 = ga; a0 < gb; a0 += ab) {
+      const int nblk = min(ab, gb - a0);
+      if (tid < nblk) {
+        const double *t = g.atab + 5 * (a0 + tid);
+        const double c = t[0], s = t[1];
+        const double p0 = Y0 * c - X0 * s, p1 = YN * c - X0 * s;
+        const double p2 = Y0 * c - XN * s, p3 = YN * c - XN * s;
+        const double tmin = fmin(fmin(p0, p1), fmin(p2, p3));
+        const double tmax = fmax(fmax(p0, p1), fmax(p2, p3));
+        const int lo = max((int)floor(tmin * g.ihp - g.hp0) - 2, 0);
+        const int hi = min((int)floor(tmax * g.ihp - g.hp0) + 2, nh - 1);
+        s_lo[tid] = lo;
+        s_cnt[tid] = max(0, min(hi - lo + 1, hmax));
       }
-      *reinterpret_cast<float4 *>(s_rows + ((size_t)t * hmax + hh) * R + 4 * q4) = v;
-    }
-    __syncthreads();
-    if (active) {
-      for (int t = 0; t < nblk; t++) {
-        const int a = a0 + t;
-        const double *tab = g.atab + 5 * a;
-        const ParFoot f = par_footprint(g, tab, x0, xn, y0, yn);
-        const int slo = s_lo[t], scnt = s_cnt[t];
-        for (int h = f.lo; h <= f.hi; h++) {
-          double w;
-          const int use = par_weight(g, tab, f, g.h_pix[h], w);
-          if (use == 0) continue;
-          if (use < 0) break;
-          const float wf = (float)w;
-          const int hh = h - slo;
-          if (hh >= 0 && hh < scnt) {
-            const float *pr = s_rows + ((size_t)t * hmax + hh) * R;
-#pragma unroll
-            for (int q4 = 0; q4 < Q4; q4++) {
-              const float4 v = *reinterpret_cast<const float4 *>(pr + 4 * q4);
-              acc[4 * q4] = acc[4 * q4] + (Acc)(v.x * wf);
-              acc[4 * q4 + 1] = acc[4 * q4 + 1] + (Acc)(v.y * wf);
-              acc[4 * q4 + 2] = acc[4 * q4 + 2] + (Acc)(v.z * wf);
-              acc[4 * q4 + 3] = acc[4 * q4 + 3] + (Acc)(v.w * wf);
-            }
+      __syncthreads();
+      const int total = nblk * hmax * Q4;
+      for (int e = tid; e < total; e += 256) {
+        const int q4 = e % Q4, r = e / Q4;
+        const int hh = r % hmax, t = r / hmax;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (hh < s_cnt[t]) {
+          const float *p = sino + ((size_t)(a0 + t) * nh + s_lo[t] + hh) * nv + k0 + 4 * q4;
+          if (aligned && 4 * q4 + 3 < nrow) {
+            __builtin_memcpy(&v, p, sizeof(v));
           } else {
-            const float *pr = sino + ((size_t)a * nh + h) * nv + k0;
+            if (4 * q4 < nrow) v.x = p[0];
+            if (4 * q4 + 1 < nrow) v.y = p[1];
+            if (4 * q4 + 2 < nrow) v.z = p[2];
+            if (4 * q4 + 3 < nrow) v.w = p[3];
+          }
+        }
+        *reinterpret_cast<float4 *>(s_rows + ((size_t)t * hmax + hh) * R + 4 * q4) = v;
+      }
+      __syncthreads();
+      if (active) {
+        for (int t = 0; t < nblk; t++) {
+          const int a = a0 + t;
+          const double *tab = g.atab + 5 * a;
+          const ParFoot f = par_footprint(g, tab, x0, xn, y0, yn);
+          const int slo = s_lo[t], scnt = s_cnt[t];
+          for (int h = f.lo; h <= f.hi; h++) {
+            double w;
+            const int use = par_weight(g, tab, f, g.h_pix[h], w);
+            if (use == 0) continue;
+            if (use < 0) break;
+            const float wf = (float)w;
+            const int hh = h - slo;
+            if (hh >= 0 && hh < scnt) {
+              const float *pr = s_rows + ((size_t)t * hmax + hh) * R;
 #pragma unroll
-            for (int q = 0; q < R; q++)
-              if (q < nrow) acc[q] = acc[q] + (Acc)(pr[q] * wf);
+              for (int q4 = 0; q4 < Q4; q4++) {
+                const float4 v = *reinterpret_cast<const float4 *>(pr + 4 * q4);
+                acc[4 * q4] = acc[4 * q4] + (Acc)(v.x * wf);
+                acc[4 * q4 + 1] = acc[4 * q4 + 1] + (Acc)(v.y * wf);
+                acc[4 * q4 + 2] = acc[4 * q4 + 2] + (Acc)(v.z * wf);
+                acc[4 * q4 + 3] = acc[4 * q4 + 3] + (Acc)(v.w * wf);
+              }
+            } else {
+              const float *pr = sino + ((size_t)a * nh + h) * nv + k0;
+#pragma unroll
+              for (int q = 0; q < R; q++)
+                if (q < nrow) acc[q] = acc[q] + (Acc)(pr[q] * wf);
+            }
           }
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
+    if (FOLD) {
+#pragma unroll
+      for (int q = 0; q < R; q++) tot[q] = grp == 0 ? acc[q] : tot[q] + acc[q];
+    }
   }
   if (!active) return 0.0;
   double ss = 0.0;
 #pragma unroll
   for (int q = 0; q < R; q++)
     if (q < nk) {
-      const float o = (float)acc[q];
+      const float o =
+          FOLD ? (float)(acc_in ? (Acc)vcol[q] + tot[q] : tot[q]) : (float)acc[q];
       vcol[q] = o;
       ss += (double)o * (double)o;
     }

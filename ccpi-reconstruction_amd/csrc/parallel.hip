@@ -227,7 +227,7 @@ static size_t par_bp_lds_bytes(const ParBpShape &S) {
 // (four per SIMD on 256 CUs).
 static ParBpShape par_bp_shape(const Geometry &G, const ParArgs &A) {
   ParBpShape S;
-  if (A.map_type != 1 || G.nv % 4 != 0) return S;
+  if (A.map_type != 1) return S;
   S.slices = G.nz <= 8 ? 8 : 16;
   const double span = std::hypot(16.0 * G.vsize[0], 16.0 * G.vsize[1]) * std::fabs(A.ihp);
   if (!(span < 4096.0)) return S;

@@ -75,9 +75,10 @@ struct DeviceBuffer {
   T *as() const { return static_cast<T *>(p); }
 };
 
-// Scaled-sinogram scratch (y * d_conv, the reference's limited_memory =
-// false copy, c2D.cpp:1105-1114) of the cone BP calls that bring none of
-// their own.  One per plan, shared by every such caller of the geometry:
+// BP scratch of the calls that bring none of their own: the cone BP's
+// scaled sinogram (y * d_conv, the reference's limited_memory = false
+// copy, c2D.cpp:1105-1114) or the parallel BP's angle-group partials.
+// One per plan, shared by every such caller of the geometry:
 // a caller holds `mu` while it enqueues the scaling and the BP, and waits
 // on the device for `done` (recorded after the previous caller's BP), so
 // calls from different streams or host threads run one after another on
@@ -114,7 +115,7 @@ struct ConePlan {
 
 // Launch shape of the tiled parallel BP (detector rows staged in LDS per
 // 16 x 16 column tile and block of angles); tiled = 0 keeps the
-// column-per-lane kernel (rows per slice other than 1, unaligned rows, or
+// column-per-lane kernel (rows per slice other than 1, or
 // footprints too wide for LDS).
 struct ParBpShape {
   int tiled = 0;

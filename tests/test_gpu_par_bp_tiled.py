@@ -1,13 +1,15 @@
 """The tiled parallel-beam backprojector against the oracle (GPU test).
 
-With one detector row per slice and rows in groups of four, the parallel BP
-stages each 16 x 16 column tile's detector columns in LDS per block of
-angles, 8 or 16 slices per lane, and splits the angles into groups (summed
-through partials) while the tiles alone would give fewer than 4096 waves.
-These geometries reach: ragged tiles and a partial last slice chunk (nz =
-20, rows beyond the volume), pixels finer than the voxels (wide staged
-spans), 8 slices per lane with many angle groups, and one angle group (a
-large volume).  Each is checked against the oracle's b2D (1e-5 of the
+With one detector row per slice, the parallel BP stages each 16 x 16
+column tile's detector columns in LDS per block of angles (16 bytes at a
+time when the row length is a multiple of 4, else one float at a time),
+8 or 16 slices per lane, and splits the angles into groups (summed through
+partials) while the tiles alone would give fewer than 4096 waves.  These
+geometries reach: ragged tiles and a partial last slice chunk (nz = 20,
+rows beyond the volume), pixels finer than the voxels (wide staged spans),
+8 slices per lane with many angle groups, rows of odd length (nz = nv =
+7, 13), and one angle group (a large volume).  Each is checked against the
+oracle's b2D (1e-5 of the
 maximum, the summation-order tolerance of the other BP tests), for the
 adjoint identity with the FP, for accumulation into a non-zero volume, and
 for run-to-run identity.
@@ -33,6 +35,12 @@ CASES = {
     "r8_groups": PB((16, 16, 4), (1.0, 1.0, 1.0), np.deg2rad(np.linspace(0, 180, 64,
                                                                         endpoint=False) + 1),
                     centred(24, 1.0, 0.37), centred(4, 1.0)),
+    "odd_rows_7": PB((21, 18, 7), (1.0, 1.0, 1.0), np.deg2rad(np.linspace(0, 180, 36,
+                                                                         endpoint=False) + 2),
+                     centred(30, 1.0, 0.21), centred(7, 1.0)),
+    "odd_rows_13": PB((40, 33, 13), (1.0, 1.0, 1.0), np.deg2rad(np.linspace(0, 180, 50,
+                                                                           endpoint=False)),
+                      centred(58, 1.0, -0.4), centred(13, 1.0)),
     "one_group": PB((512, 512, 16), (1.0, 1.0, 1.0), np.deg2rad(np.linspace(0, 180, 60,
                                                                            endpoint=False)),
                     centred(725, 1.0), centred(16, 1.0)),

@@ -7,14 +7,19 @@ S in [1, 32], 90 / 180 / 360 angles, width ceil(sqrt2 N), 1 % Gaussian
 noise), 20 iterations:
   * the batched form (one persistent launch per step, graph replay) against
     the same problems one at a time through ccpi_cgls: iterates within
-    1e-4 of the maximum and norms within 1e-4, per problem (the one-problem
-    path's tiled backprojector adds angle groups' partial sums, the batched
-    one sums each column's angles in order: the same terms in another
-    order, whose rounding 20 iterations amplify to ~2e-5);
+    1e-6 of the maximum and norms within 1e-6 of the first, per problem
+    (the batched items run the one-problem kernels' bodies, and its tiled
+    backprojector items add the one-problem launch's angle groups in the
+    same order, so the floats are the same; 20 iterations would amplify a
+    different summation order to ~1e-4);
   * the adjoint identity <A x, y> = <x, A^T y> of every problem's projector;
   * the N = 128 problems against the FP64 oracle (the same model in double,
     cgls.cpp:26-157): the device iterate is no further from it than the
-    float oracle's (the reference's arithmetic) is, plus 1e-4 relative.
+    float oracle's (the reference's arithmetic) is, within 1 % of that
+    distance per problem and on average over the problems.  Both are float
+    evaluations of 20 noisy-data iterations, 2.5 - 5 % from the FP64 iterate;
+    another summation order of the same float terms moves that distance by
+    a few tenths of a percent either way.
 
 Differentiable cone reconstruction, 256^3 of 1 mm, 360 projections over
 360 deg, 384^2 detector of 2 mm, SOD 500, SDD 1000: loss
@@ -73,8 +78,8 @@ def test_batched_64_matches_one_at_a_time(batch64):
         ndev = float(np.max(np.abs(nb_i - np_i)) / np_i[0])
         print(f"problem {i} {g.shape}: iterate {dev:.2e}, norms {ndev:.2e} of ||A^T b||, "
               f"last norm {np_i[-1] / np_i[0]:.2e} of it", flush=True)
-        assert dev <= 1e-4, (i, g.shape, dev)
-        assert ndev <= 1e-4, (i, g.shape, ndev)
+        assert dev <= 1e-6, (i, g.shape, dev)
+        assert ndev <= 1e-6, (i, g.shape, ndev)
     print(f"max deviation batched vs one at a time {worst:.2e}", flush=True)
 
 
@@ -96,6 +101,7 @@ def test_batched_128_problems_against_fp64_oracle(batch64):
     idx = [i for i, g in enumerate(geoms) if g.shape[0] == 128]
     assert len(idx) >= 4
     xb, _, _ = cgls_batched([geoms[i] for i in idx], [bs[i] for i in idx], 20)
+    devs, refs = [], []
     for q, i in enumerate(idx):
         g = geoms[i]
         og = oracle.Geometry(g.angles, g.h_pixels, g.v_pixels, g.origin, g.voxel_size,
@@ -108,7 +114,12 @@ def test_batched_128_problems_against_fp64_oracle(batch64):
         e_ref = np.linalg.norm(x32 - x64) / nrm
         print(f"problem {i} {g.shape} {g.angles.size} angles: distance to FP64 "
               f"device {e_dev:.3e} float oracle {e_ref:.3e}", flush=True)
-        assert e_dev <= e_ref + 1e-4
+        assert e_dev <= 1.01 * e_ref
+        devs.append(e_dev)
+        refs.append(e_ref)
+    print(f"mean distance to FP64: device {np.mean(devs):.4e} float oracle "
+          f"{np.mean(refs):.4e}", flush=True)
+    assert np.mean(devs) <= np.mean(refs)
 
 
 # ---- differentiable cone reconstruction at 256^3 ---------------------------
