@@ -8,7 +8,8 @@
 //   FP    items (problem, 256/lr-ray block, 4*lr-row block) -> Ad, partial ||Ad||^2
 //   alpha one workgroup per problem: fixed-order sum of its partials
 //   x/r   items (problem, element chunk)               -> x += a d, r -= a Ad
-//   BP    items (problem, 256-column block, slice block) -> s, partial ||s||^2
+//   BP    items (problem, 16x16-column tile or 256-column block, slice
+//         block)                                    -> s, partial ||s||^2
 //   beta  one workgroup per problem
 //   d     items (problem, element chunk)               -> d = s + b d
 // The six launches are captured once in a hipGraph and replayed per
@@ -89,7 +90,10 @@ __device__ inline double wave_sum_d(double v) {
 // so the sums are those of the one-problem launch) or, for problems the
 // tiled form does not take, 256 columns of the column-per-lane body.  Every
 // quarter (FP) or wave (BP) of an item leaves one FP64 partial norm.
-template <int RC, int MODE>
+// BPK: the BP bodies a batch needs (1 tiled, 2 column-per-lane, 3 both);
+// a kernel holding both bodies is sized for the larger of their register
+// sets, so a batch of one kind gets the kernel of that body alone.
+template <int RC, int MODE, int BPK>
 __global__ __launch_bounds__(256) void batch_proj_kernel(BatchProb *__restrict__ P,
                                                          BatchLists L) {
   const int *off = MODE == 0 ? L.fp_off : L.bp_off;
@@ -125,14 +129,15 @@ __global__ __launch_bounds__(256) void batch_proj_kernel(BatchProb *__restrict__
     const BatchProb &B = P[p];
     const ParArgs g = B.g;
     const int local = item - off[p];
+    const float *r = B.r;
     float *dst = MODE == 1 ? B.s : B.d;
     double ss;
-    if (B.bp_tiled) {
+    if (BPK == 1 || (BPK == 3 && B.bp_tiled)) {
       ss = par_bp_tile<RC, float, true>(g, local % B.bp_nbx, local / B.bp_nbx, 0, g.na,
-                                        B.bp_hmax, B.bp_ab, false, B.r, dst, smem,
+                                        B.bp_hmax, B.bp_ab, false, r, dst, smem,
                                         B.bp_groups);
     } else {
-      ss = par_bp_thread<RC>(g, local % B.bp_nbx, local / B.bp_nbx, threadIdx.x, B.r, dst,
+      ss = par_bp_thread<RC>(g, local % B.bp_nbx, local / B.bp_nbx, threadIdx.x, r, dst,
                              false);
     }
     ss = wave_sum_d(ss);
@@ -227,31 +232,43 @@ int blocks_per_cu(K kernel, size_t lds) {
 }
 
 // lds: dynamic LDS of the BP launches (the tiled items' staged rows)
-template <int RC, int MODE>
+template <int RC, int MODE, int BPK>
 void launch_one(int grid, size_t lds, BatchProb *P, const BatchLists &L, hipStream_t st) {
   const size_t l = MODE == 0 ? 0 : lds;
-  const int per_cu = blocks_per_cu(batch_proj_kernel<RC, MODE>, l);
+  const int per_cu = blocks_per_cu(batch_proj_kernel<RC, MODE, BPK>, l);
   const int g = std::max(1, std::min(grid, per_cu * cu_count()));
-  hipLaunchKernelGGL((batch_proj_kernel<RC, MODE>), dim3(g), dim3(256), l, st, P, L);
+  hipLaunchKernelGGL((batch_proj_kernel<RC, MODE, BPK>), dim3(g), dim3(256), l, st, P, L);
+}
+
+template <int RC, int MODE>
+void launch_bp(int bpk, int grid, size_t lds, BatchProb *P, const BatchLists &L,
+               hipStream_t st) {
+  if (bpk == 1)
+    launch_one<RC, MODE, 1>(grid, lds, P, L, st);
+  else if (bpk == 2)
+    launch_one<RC, MODE, 2>(grid, lds, P, L, st);
+  else
+    launch_one<RC, MODE, 3>(grid, lds, P, L, st);
 }
 
 template <int RC>
-void launch_proj(int mode, int grid, size_t lds, BatchProb *P, const BatchLists &L,
+void launch_proj(int mode, int bpk, int grid, size_t lds, BatchProb *P, const BatchLists &L,
                  hipStream_t st) {
   if (mode == 0)
-    launch_one<RC, 0>(grid, lds, P, L, st);
+    launch_one<RC, 0, 3>(grid, lds, P, L, st);  // the FP has no BP body
   else if (mode == 1)
-    launch_one<RC, 1>(grid, lds, P, L, st);
+    launch_bp<RC, 1>(bpk, grid, lds, P, L, st);
   else
-    launch_one<RC, 2>(grid, lds, P, L, st);
+    launch_bp<RC, 2>(bpk, grid, lds, P, L, st);
 }
 
-void proj(int rc, int mode, int grid, size_t lds, BatchProb *P, const BatchLists &L,
+// bpk: BP bodies the batch needs (BatchRun::bp_kinds)
+void proj(int rc, int mode, int bpk, int grid, size_t lds, BatchProb *P, const BatchLists &L,
           hipStream_t st) {
   if (rc == 8)
-    launch_proj<8>(mode, grid, lds, P, L, st);
+    launch_proj<8>(mode, bpk, grid, lds, P, L, st);
   else
-    launch_proj<16>(mode, grid, lds, P, L, st);
+    launch_proj<16>(mode, bpk, grid, lds, P, L, st);
   CCPI_HIP(hipGetLastError());
 }
 
@@ -263,6 +280,7 @@ struct BatchRun {
   int count = 0;
   int fp_total = 0, bp_total = 0;
   size_t bp_lds = 0;   // dynamic LDS of the BP launches
+  int bp_kinds = 0;    // BP bodies of the problems: 1 tiled, 2 column-per-lane
 };
 
 // Item lists for row/slice blocks of `rc`.  A problem whose plan takes the
@@ -271,6 +289,7 @@ struct BatchRun {
 void layout(BatchRun &R, int rc, hipStream_t st) {
   std::vector<int> off(4 * (R.count + 1), 0);
   R.bp_lds = 0;
+  R.bp_kinds = 0;
   for (int p = 0; p < R.count; p++) {
     BatchProb &B = R.host[p];
     const ParArgs &g = B.g;
@@ -279,6 +298,7 @@ void layout(BatchRun &R, int rc, hipStream_t st) {
     B.fp_nbx = (g.na * g.nh + 256 / B.fp_lr - 1) / (256 / B.fp_lr);
     B.fp_nby = (g.nv + 4 * B.fp_lr - 1) / (4 * B.fp_lr);
     B.bp_tiled = S.tiled;
+    R.bp_kinds |= S.tiled ? 1 : 2;
     if (S.tiled) {
       const size_t per_angle = sizeof(float) * (size_t)S.hmax * rc + 2 * sizeof(int);
       B.bp_hmax = S.hmax;
@@ -395,10 +415,10 @@ void run_pb_cgls_batched(const std::vector<Geometry> &geoms, float *const *b,
         const int grid = mult * cu_count();
         // FP of d (scratch, contents irrelevant) and BP of r into s: the
         // two projector launches of an iteration, outputs discarded
-        proj(rc, 0, grid, R.bp_lds, R.probs.as<BatchProb>(), R.L, st);
+        proj(rc, 0, R.bp_kinds, grid, R.bp_lds, R.probs.as<BatchProb>(), R.L, st);
         CCPI_HIP(hipEventRecord(e0, st));
-        proj(rc, 0, grid, R.bp_lds, R.probs.as<BatchProb>(), R.L, st);
-        proj(rc, 1, grid, R.bp_lds, R.probs.as<BatchProb>(), R.L, st);
+        proj(rc, 0, R.bp_kinds, grid, R.bp_lds, R.probs.as<BatchProb>(), R.L, st);
+        proj(rc, 1, R.bp_kinds, grid, R.bp_lds, R.probs.as<BatchProb>(), R.L, st);
         CCPI_HIP(hipEventRecord(e1, st));
         CCPI_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -419,7 +439,7 @@ void run_pb_cgls_batched(const std::vector<Geometry> &geoms, float *const *b,
   const int vgrid = 4 * cu_count();
 
   // start: d = A^T b, gamma = ||d||^2 (cgls.cpp:45-49)
-  proj(T.rc, 2, T.grid, R.bp_lds, P, R.L, st);
+  proj(T.rc, 2, R.bp_kinds, T.grid, R.bp_lds, P, R.L, st);
   hipLaunchKernelGGL(batch_scalar_kernel<0>, dim3(R.count), dim3(256), 0, st, P, R.L);
   CCPI_HIP(hipGetLastError());
 
@@ -427,10 +447,10 @@ void run_pb_cgls_batched(const std::vector<Geometry> &geoms, float *const *b,
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   CCPI_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  proj(T.rc, 0, T.grid, R.bp_lds, P, R.L, st);
+  proj(T.rc, 0, R.bp_kinds, T.grid, R.bp_lds, P, R.L, st);
   hipLaunchKernelGGL(batch_scalar_kernel<1>, dim3(R.count), dim3(256), 0, st, P, R.L);
   hipLaunchKernelGGL(batch_vec_kernel<0>, dim3(vgrid), dim3(256), 0, st, P, R.L);
-  proj(T.rc, 1, T.grid, R.bp_lds, P, R.L, st);
+  proj(T.rc, 1, R.bp_kinds, T.grid, R.bp_lds, P, R.L, st);
   hipLaunchKernelGGL(batch_scalar_kernel<2>, dim3(R.count), dim3(256), 0, st, P, R.L);
   hipLaunchKernelGGL(batch_vec_kernel<1>, dim3(vgrid), dim3(256), 0, st, P, R.L);
   CCPI_HIP(hipStreamEndCapture(st, &graph));

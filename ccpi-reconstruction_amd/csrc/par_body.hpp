@@ -9,14 +9,34 @@
 
 namespace ccpi {
 
+// Device memory reached through a pointer typed as global: loads and stores
+// compile to global (not flat) accesses.  The batched kernels read their
+// problems' pointers from descriptors in device memory, which leaves them
+// generic to the compiler; generic accesses wait on the scalar and vector
+// memory counters both and need 64-bit addresses in vector registers.  The
+// bodies below reach all their arrays through gmem.
+#define CCPI_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ inline CCPI_GLOBAL T *gmem(T *p) {
+  return (CCPI_GLOBAL T *)p;
+}
+
+// 16 bytes at p (4-byte aligned) as one load.
+__device__ inline float4 load_f4(const CCPI_GLOBAL float *p) {
+  float4 v;
+  __builtin_memcpy(&v, p, sizeof(v));
+  return v;
+}
+
 // The detector line of ray (a, h) walked through the xy grid (p2D.cpp:
 // 215-458): add(x, y, l) for every crossed cell, l its xy length.
 template <class Add>
 __device__ inline void par_walk(const ParArgs &g, int a, int h, Add &&add) {
   const int nx = g.nx, ny = g.ny;
-  const double c = g.atab[5 * a], s = g.atab[5 * a + 1];
-  const double p2x = c * g.D - s * g.h_pix[h];
-  const double p2y = s * g.D + c * g.h_pix[h];
+  const double c = gmem(g.atab)[5 * a], s = gmem(g.atab)[5 * a + 1];
+  const double hp = gmem(g.h_pix)[h];
+  const double p2x = c * g.D - s * hp;
+  const double p2y = s * g.D + c * hp;
   if (fabs(c) < kEps) {
     if (fabs(s) >= kEps) {  // line parallel to y (p2D.cpp:233-260)
       const int i = (int)floor((p2x - g.ox) / g.dx);
@@ -100,9 +120,10 @@ __device__ inline void par_walk(const ParArgs &g, int a, int h, Add &&add) {
 template <int G, class Emit>
 __device__ inline void par_walk_groups(const ParArgs &g, int a, int h, Emit &&emit) {
   const int nx = g.nx, ny = g.ny;
-  const double c = g.atab[5 * a], s = g.atab[5 * a + 1];
-  const double p2x = c * g.D - s * g.h_pix[h];
-  const double p2y = s * g.D + c * g.h_pix[h];
+  const double c = gmem(g.atab)[5 * a], s = gmem(g.atab)[5 * a + 1];
+  const double hp = gmem(g.h_pix)[h];
+  const double p2x = c * g.D - s * hp;
+  const double p2y = s * g.D + c * hp;
   int cx[G], cy[G];
   float l[G];
   if (fabs(c) < kEps) {
@@ -216,14 +237,14 @@ __device__ inline double par_fp_rows(const ParArgs &g, int r, int v0,
   static_assert(RC % 4 == 0, "rows are read in groups of four");
   const int a = r / g.nh, h = r - a * g.nh;
   const int nz = g.nz, ny = g.ny;
-  float *__restrict__ out = sino + (size_t)r * g.nv + v0;
+  CCPI_GLOBAL float *__restrict__ out = gmem(sino) + (size_t)r * g.nv + v0;
   const int nrow = min(RC, g.nv - v0);
   Acc acc[RC];
   int kmap[RC];
 #pragma unroll
   for (int q = 0; q < RC; q++) {
     acc[q] = (acc_in && q < nrow) ? (Acc)out[q] : (Acc)0;
-    const int k = q < nrow ? g.map[v0 + q] : -1;
+    const int k = q < nrow ? gmem(g.map)[v0 + q] : -1;
     kmap[q] = (k >= 0 && k < nz) ? k : -1;
   }
   // one row per slice (map_type 1): the rows are the contiguous slices
@@ -239,13 +260,14 @@ __device__ inline double par_fp_rows(const ParArgs &g, int r, int v0,
       float4 f[kParFpGroup][Q4];
 #pragma unroll
       for (int q = 0; q < kParFpGroup; q++) {
-        const float *__restrict__ cv = vol + ((size_t)cx[q] * ny + cy[q]) * nz + v0;
+        const CCPI_GLOBAL float *__restrict__ cv =
+            gmem(vol) + ((size_t)cx[q] * ny + cy[q]) * nz + v0;
 #pragma unroll
         for (int q4 = 0; q4 < Q4; q4++) {
           float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
           if (q < n) {
             if (4 * q4 + 3 < nval) {
-              __builtin_memcpy(&v, cv + 4 * q4, sizeof(v));
+              v = load_f4(cv + 4 * q4);
             } else {
               if (4 * q4 < nval) v.x = cv[4 * q4];
               if (4 * q4 + 1 < nval) v.y = cv[4 * q4 + 1];
@@ -271,7 +293,7 @@ __device__ inline double par_fp_rows(const ParArgs &g, int r, int v0,
     });
   } else {
     par_walk(g, a, h, [&](int x, int y, float l) {
-      const float *__restrict__ col = vol + ((size_t)x * ny + y) * nz;
+      const CCPI_GLOBAL float *__restrict__ col = gmem(vol) + ((size_t)x * ny + y) * nz;
 #pragma unroll
       for (int q = 0; q < RC; q++)
         if (kmap[q] >= 0) acc[q] = acc[q] + (Acc)(col[kmap[q]] * l);
@@ -331,7 +353,8 @@ struct ParFoot {
   double ymin, ymax, lim0, lim1;
 };
 
-__device__ inline ParFoot par_footprint(const ParArgs &g, const double *t, double x0, double xn,
+template <class TabPtr>
+__device__ inline ParFoot par_footprint(const ParArgs &g, TabPtr t, double x0, double xn,
                                         double y0, double yn) {
   const double c = t[0], s = t[1];
   ParFoot f;
@@ -372,7 +395,7 @@ __device__ inline ParFoot par_footprint(const ParArgs &g, const double *t, doubl
       else         { f.ymin = yn * c - x0 * s; f.ymax = y0 * c - xn * s; }
     }
     // clamp at 0: the floor can round to -1 when ymin ~ h_pix[0]
-    f.lo = f.ymin < g.h_pix[0] ? 0 : max((int)floor(f.ymin * g.ihp - g.hp0), 0);
+    f.lo = f.ymin < gmem(g.h_pix)[0] ? 0 : max((int)floor(f.ymin * g.ihp - g.hp0), 0);
     f.hi = min((int)floor(f.ymax * g.ihp - g.hp0), g.nh - 1);
   }
   return f;
@@ -380,7 +403,8 @@ __device__ inline ParFoot par_footprint(const ParArgs &g, const double *t, doubl
 
 // Weight of detector column position hp in footprint f (p2D.cpp:916-938):
 // 1 with w set, 0 to skip the column, -1 past the footprint's far edge.
-__device__ inline int par_weight(const ParArgs &g, const double *t, const ParFoot &f, double hp,
+template <class TabPtr>
+__device__ inline int par_weight(const ParArgs &g, TabPtr t, const ParFoot &f, double hp,
                                  double &w) {
   if (f.kind != 0) {
     const double tt = f.kind == 1 ? hp : -hp;
@@ -415,7 +439,7 @@ __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid
   const int k0 = by * RC;
   const int nz = g.nz, nv = g.nv, nh = g.nh;
   const int nk = min(RC, nz - k0);
-  float *__restrict__ vcol = vol + (size_t)colid * nz + k0;
+  CCPI_GLOBAL float *__restrict__ vcol = gmem(vol) + (size_t)colid * nz + k0;
   Acc acc[RC];
 #pragma unroll
   for (int q = 0; q < RC; q++) acc[q] = (acc_in && q < nk) ? (Acc)vcol[q] : (Acc)0;
@@ -425,7 +449,7 @@ __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid
   const int nvalid = max(0, min(nk, nv - k0));  // rows of the slices (T == 1)
 
   for (int a = 0; a < g.na; a++) {
-    const double *t = g.atab + 5 * a;
+    const CCPI_GLOBAL double *t = gmem(g.atab) + 5 * a;
     const double c = t[0], s = t[1];
     int lo, hi, kind;
     double ymin = 0, ymax = 0, lim0 = 0, lim1 = 0;
@@ -465,11 +489,11 @@ __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid
         else         { ymin = yn * c - x0 * s; ymax = y0 * c - xn * s; }
       }
       // clamp at 0: the floor can round to -1 when ymin ~ h_pix[0]
-      lo = ymin < g.h_pix[0] ? 0 : max((int)floor(ymin * g.ihp - g.hp0), 0);
+      lo = ymin < gmem(g.h_pix)[0] ? 0 : max((int)floor(ymin * g.ihp - g.hp0), 0);
       hi = min((int)floor(ymax * g.ihp - g.hp0), nh - 1);
     }
     for (int h = lo; h <= hi; h++) {
-      const double hp = g.h_pix[h];
+      const double hp = gmem(g.h_pix)[h];
       double w;
       if (kind != 0) {
         const double tt = kind == 1 ? hp : -hp;
@@ -488,15 +512,14 @@ __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid
           break;
       }
       const float wf = (float)w;
-      const float *__restrict__ p = sino + ((size_t)a * nh + h) * nv;
+      const CCPI_GLOBAL float *__restrict__ p = gmem(sino) + ((size_t)a * nh + h) * nv;
       if (T == 1) {
         // rows k0 .. k0 + nvalid - 1 of the ray, read 16 bytes at a time
-        const float *__restrict__ pr = p + k0;
+        const CCPI_GLOBAL float *__restrict__ pr = p + k0;
 #pragma unroll
         for (int q4 = 0; q4 < RC / 4; q4++) {
           if (4 * q4 + 3 < nvalid) {
-            float4 f;
-            __builtin_memcpy(&f, pr + 4 * q4, sizeof(f));
+            const float4 f = load_f4(pr + 4 * q4);
             acc[4 * q4] = acc[4 * q4] + (Acc)(f.x * wf);
             acc[4 * q4 + 1] = acc[4 * q4 + 1] + (Acc)(f.y * wf);
             acc[4 * q4 + 2] = acc[4 * q4 + 2] + (Acc)(f.z * wf);
@@ -522,7 +545,8 @@ __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid
         for (int q = 0; q < RC; q++) {
           if (q < nk) {
             const int k = k0 + q;
-            for (int v = g.rlo[k]; v < g.rhi[k]; v++) acc[q] = acc[q] + (Acc)(p[v] * wf);
+            const int vlo = gmem(g.rlo)[k], vhi = gmem(g.rhi)[k];
+            for (int v = vlo; v < vhi; v++) acc[q] = acc[q] + (Acc)(p[v] * wf);
           }
         }
       }
@@ -552,16 +576,19 @@ __device__ inline double par_bp_thread(const ParArgs &g, int bx, int by, int tid
 // one angle group the result is bit-identical to par_bp_kernel; with G
 // groups (blockIdx.z) each group writes its partial sums of its angle range
 // to part[group] and par_bp_sum_kernel adds them in group order.
+//
 // One tile item of the tiled BP, run by all 256 threads of a workgroup
 // (it synchronises the workgroup): tile `tile` (16 x 16 columns, row-major
 // over the column grid), slices [zc * R, zc * R + R), angles [a_begin,
 // a_end).  The result goes to `dst` (a volume-layout array), added to its
 // contents when acc_in.  Returns the thread's sum of squares of the values
 // it wrote (0 for threads outside the volume).  smem: par_bp_lds_bytes.
-// FOLD: the angles are split into `groups` ranges as par_bp_lds_kernel
-// splits them, each summed from zero and the sums added in group order in
-// registers, so that one item gives bit for bit what the grouped launch
-// and par_bp_sum_kernel give (the batched kernel's tiled items).
+// FOLD (the batched kernel's tiled items; acc_in must be false): the
+// angles are split into `groups` ranges as par_bp_lds_kernel splits them,
+// each summed from zero, and the running sum of the groups is kept in
+// `dst` (each lane re-reads only its own writes), so one item gives bit for
+// bit what the grouped launch and par_bp_sum_kernel give into a zeroed
+// volume, without registers for a second set of sums.
 template <int R, class Acc, bool FOLD = false>
 __device__ double par_bp_tile(const ParArgs &g, int tile, int zc, int a_begin, int a_end,
                               int hmax, int ab, bool acc_in, const float *__restrict__ sino,
@@ -580,11 +607,10 @@ __device__ double par_bp_tile(const ParArgs &g, int tile, int zc, int a_begin, i
   const int nk = min(R, nz - k0);
   const size_t colid = (size_t)(active ? i : 0) * g.ny + (active ? j : 0);
   Acc acc[R];
-  float *__restrict__ vcol = dst_vol + colid * nz + k0;
+  CCPI_GLOBAL float *__restrict__ vcol = gmem(dst_vol) + colid * nz + k0;
 #pragma unroll
   for (int q = 0; q < R; q++)
     acc[q] = (!FOLD && acc_in && active && q < nk) ? (Acc)vcol[q] : (Acc)0;
-  Acc tot[R];  // FOLD: sum of the groups done so far
   const double x0 = g.ox + (double)i * g.dx, xn = g.ox + (double)(i + 1) * g.dx;
   const double y0 = g.oy + (double)j * g.dy, yn = g.oy + (double)(j + 1) * g.dy;
   // tile square
@@ -608,7 +634,7 @@ __device__ double par_bp_tile(const ParArgs &g, int tile, int zc, int a_begin, i
     for (int a0 = ga; a0 < gb; a0 += ab) {
       const int nblk = min(ab, gb - a0);
       if (tid < nblk) {
-        const double *t = g.atab + 5 * (a0 + tid);
+        const CCPI_GLOBAL double *t = gmem(g.atab) + 5 * (a0 + tid);
         const double c = t[0], s = t[1];
         const double p0 = Y0 * c - X0 * s, p1 = YN * c - X0 * s;
         const double p2 = Y0 * c - XN * s, p3 = YN * c - XN * s;
@@ -626,9 +652,10 @@ __device__ double par_bp_tile(const ParArgs &g, int tile, int zc, int a_begin, i
         const int hh = r % hmax, t = r / hmax;
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
         if (hh < s_cnt[t]) {
-          const float *p = sino + ((size_t)(a0 + t) * nh + s_lo[t] + hh) * nv + k0 + 4 * q4;
+          const CCPI_GLOBAL float *p =
+              gmem(sino) + ((size_t)(a0 + t) * nh + s_lo[t] + hh) * nv + k0 + 4 * q4;
           if (aligned && 4 * q4 + 3 < nrow) {
-            __builtin_memcpy(&v, p, sizeof(v));
+            v = load_f4(p);
           } else {
             if (4 * q4 < nrow) v.x = p[0];
             if (4 * q4 + 1 < nrow) v.y = p[1];
@@ -642,12 +669,12 @@ __device__ double par_bp_tile(const ParArgs &g, int tile, int zc, int a_begin, i
       if (active) {
         for (int t = 0; t < nblk; t++) {
           const int a = a0 + t;
-          const double *tab = g.atab + 5 * a;
+          const CCPI_GLOBAL double *tab = gmem(g.atab) + 5 * a;
           const ParFoot f = par_footprint(g, tab, x0, xn, y0, yn);
           const int slo = s_lo[t], scnt = s_cnt[t];
           for (int h = f.lo; h <= f.hi; h++) {
             double w;
-            const int use = par_weight(g, tab, f, g.h_pix[h], w);
+            const int use = par_weight(g, tab, f, gmem(g.h_pix)[h], w);
             if (use == 0) continue;
             if (use < 0) break;
             const float wf = (float)w;
@@ -663,7 +690,7 @@ __device__ double par_bp_tile(const ParArgs &g, int tile, int zc, int a_begin, i
                 acc[4 * q4 + 3] = acc[4 * q4 + 3] + (Acc)(v.w * wf);
               }
             } else {
-              const float *pr = sino + ((size_t)a * nh + h) * nv + k0;
+              const CCPI_GLOBAL float *pr = gmem(sino) + ((size_t)a * nh + h) * nv + k0;
 #pragma unroll
               for (int q = 0; q < R; q++)
                 if (q < nrow) acc[q] = acc[q] + (Acc)(pr[q] * wf);
@@ -673,9 +700,11 @@ __device__ double par_bp_tile(const ParArgs &g, int tile, int zc, int a_begin, i
       }
       __syncthreads();
     }
-    if (FOLD) {
+    if (FOLD && active && grp + 1 < ngrp) {
+      // running sum of the groups so far (the last group's is added below)
 #pragma unroll
-      for (int q = 0; q < R; q++) tot[q] = grp == 0 ? acc[q] : tot[q] + acc[q];
+      for (int q = 0; q < R; q++)
+        if (q < nk) vcol[q] = grp == 0 ? (float)acc[q] : (float)((Acc)vcol[q] + acc[q]);
     }
   }
   if (!active) return 0.0;
@@ -684,7 +713,7 @@ __device__ double par_bp_tile(const ParArgs &g, int tile, int zc, int a_begin, i
   for (int q = 0; q < R; q++)
     if (q < nk) {
       const float o =
-          FOLD ? (float)(acc_in ? (Acc)vcol[q] + tot[q] : tot[q]) : (float)acc[q];
+          (FOLD && groups > 1) ? (float)((Acc)vcol[q] + acc[q]) : (float)acc[q];
       vcol[q] = o;
       ss += (double)o * (double)o;
     }
