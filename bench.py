@@ -61,12 +61,12 @@ def source_stamp() -> str:
     return h.hexdigest()[:16]
 
 
-def measured_traffic(kernel: str):
-    """HBM-side bytes per launch of `kernel` from the committed counter
-    summary (FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy): one
-    `TRAFFIC <kernel> <bytes>` line per kernel and a `SOURCES <stamp>` line.
-    Returns (bytes or None, "current" / "stale" / "missing")."""
-    path = os.path.join(ROOT, "profiles", "cone512_projectors_counters.md")
+def measured_traffic(kernel: str, summary: str = "cone512_projectors_counters.md"):
+    """HBM-side bytes per launch of `kernel` from a committed counter summary
+    under profiles/ (FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy):
+    one `TRAFFIC <kernel> <bytes>` line per kernel and a `SOURCES <stamp>`
+    line.  Returns (bytes or None, "current" / "stale" / "missing")."""
+    path = os.path.join(ROOT, "profiles", summary)
     val, stamp = None, None
     try:
         with open(path) as f:
@@ -317,14 +317,16 @@ def parallel_cfg1(cpu: bool):
     fp_ms, nf, bp_ms, nb = s.timings()
     s.close()
     fp_ms, bp_ms = fp_ms / max(nf, 1), bp_ms / max(nb, 1)
-    dom, dom_ms = ("par_bp_kernel", bp_ms) if bp_ms >= fp_ms else ("par_fp_kernel", fp_ms)
+    dom, dom_ms = ("par_bp_lds_kernel", bp_ms) if bp_ms >= fp_ms else ("par_fp_kernel", fp_ms)
     ach = bytes_alg / (dom_ms * 1e-3) / 1e9
+    traffic, traffic_state = measured_traffic(dom, "parallel_256x256x16_cgls.md")
     out = {"workload": "parallel-beam CGLS 256x256x16, 180 angles, 367x16 detector, "
                        "20 iterations, noise-free",
            "iter_s": 20.0 / secs, "ms_per_iter": secs / 20 * 1e3,
            "rmse_vs_phantom": rmse, "nnz": nnz, "fp_ms": fp_ms, "bp_ms": bp_ms,
            "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "bytes_alg": bytes_alg,
+                        "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                        "traffic_source": traffic_state, "bytes_alg": bytes_alg,
                         "fp_frac": bytes_alg / (fp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                         "bp_frac": bytes_alg / (bp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
     if cpu:
