@@ -63,7 +63,8 @@ struct alignas(32) BpCol {
   int n;          // number of records
   int min_v;      // rows >= min_v stay above the bottom for every ray
   int max_v;      // rows < max_v stay below the top for every ray
-  int pad[3];
+  float amin;     // smallest walk parameter at which a ray enters the column
+  int pad[2];
 };
 
 // One record (16 bytes): x = ray (a * nh + h), y / z = alpha interval of
@@ -80,6 +81,8 @@ struct ConeArgs {
   int bp_main_safe;      // source inside the volume's z range
   int bp_phased;         // BP accumulates in one array, in lane phases
   int bp_phases;         // lane phases (lanes of one phase on distinct slices)
+  float bp_rps1;         // voxel z size / smallest row pitch: rows per slice at
+                         // walk parameter 1
   int nvp;               // row stride of the BP's scaled sinogram (padded)
   float zb, zt;          // volume bottom / top relative to source z
   const float *alim;     // [nv] walk parameter where row v leaves the volume's

@@ -572,8 +572,9 @@ __device__ inline ColBox col_box(const ConeArgs &g, int colid) {
 }
 
 // Plan pass 1, one wavefront per column, lanes over angles: the number of
-// (angle, h) rays crossing the column and the row-set bounds min_v / max_v
-// (c2D.cpp:657-676).
+// (angle, h) rays crossing the column, the row-set bounds min_v / max_v
+// (c2D.cpp:657-676) and the smallest entry walk parameter (the column's
+// lane phases in the phased BP).
 __global__ __launch_bounds__(256) void cone_bpcount_kernel(ConeArgs g, BpCol *__restrict__ cols) {
   const int lane = threadIdx.x & 63;
   const int colid = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -581,6 +582,7 @@ __global__ __launch_bounds__(256) void cone_bpcount_kernel(ConeArgs g, BpCol *__
   const ColBox B = col_box(g, colid);
   const float pzdeps = g.pzdv * (float)(1.0 + kEps);
   int lo_max = 0, hi_min = g.nv, cnt = 0;
+  float amin = 3.0e38f;
   for (int a = lane; a < g.na; a += 64) {
     const double *t = g.atab + 8 * a;
     int hmin, hmax;
@@ -590,6 +592,7 @@ __global__ __launch_bounds__(256) void cone_bpcount_kernel(ConeArgs g, BpCol *__
       if (cone_col_alpha(g, t, h, B.x0, B.xn, B.y0, B.yn, a0, a1)) {
         lo_max = max(lo_max, (int)ceilf(pzdeps + g.z1 / a0));
         hi_min = min(hi_min, (int)floorf(g.pzdv + g.znm / a0));
+        amin = fminf(amin, a0);
         ++cnt;
       }
     }
@@ -598,6 +601,7 @@ __global__ __launch_bounds__(256) void cone_bpcount_kernel(ConeArgs g, BpCol *__
   for (int d = 32; d >= 1; d >>= 1) {
     lo_max = max(lo_max, __shfl_xor(lo_max, d, 64));
     hi_min = min(hi_min, __shfl_xor(hi_min, d, 64));
+    amin = fminf(amin, __shfl_xor(amin, d, 64));
     cnt += __shfl_xor(cnt, d, 64);
   }
   if (lane == 0) {
@@ -605,6 +609,7 @@ __global__ __launch_bounds__(256) void cone_bpcount_kernel(ConeArgs g, BpCol *__
     c.n = cnt;
     c.min_v = lo_max;
     c.max_v = hi_min;
+    c.amin = amin;
     cols[colid] = c;
   }
 }
@@ -705,11 +710,12 @@ __device__ inline BpSlot bp_slot(const ConeArgs &g, int v, int pass, bool one_se
 // over distinct cells, issued together, followed by their second-slice
 // updates.  Cells 0 and nz + 1
 // of an array are trash for slices outside the volume.  PHASED (rows per
-// slice beyond 4, or arrays beyond the LDS budget): S = 4, 8 or 16 slots
+// slice beyond 4, or arrays beyond the LDS budget): S = 4, 8, 12 or 16 slots
 // and one array; the slots update one after another, each a read-add-write
-// round, and within a slot the lanes go in g.bp_phases phases (lane L in
-// phase L mod phases), whose lanes are phases * S >= rows per slice rows
-// apart, so on distinct slices.  (LDS float atomics on one array made the
+// round, and within a slot the lanes go in phases (lane L in phase L mod
+// phases), whose lanes are phases * S >= rows per slice rows apart, so on
+// distinct slices.  The phases are the column's own (bp_col_phases), at
+// most the layout's g.bp_phases.  (LDS float atomics on one array made the
 // kernel twice as slow as these rounds, and rows interleaved over passes
 // re-read each record's detector row once per pass.)
 // At most 64 VGPRs (8 waves per SIMD) for strides 1 and 2: the only
@@ -717,6 +723,19 @@ __device__ inline BpSlot bp_slot(const ConeArgs &g, int v, int pass, bool one_se
 // hide the LDS read-add-write latency (S = 2 at 512^3: 98 -> 93 ms).  The
 // wider strides keep the registers their slots need.
 constexpr int bp_min_waves(int S) { return S <= 2 ? 8 : 1; }
+
+// Lane phases of the phased form for one column.  Its rays enter it at walk
+// parameter >= C.amin, where adjacent rows are amin * pitch apart in z, so
+// at most 1.05 * bp_rps1 / amin rows share a slice in it: the layout's
+// bound (cone_bp_layout) taken over the column instead of the volume.
+// Columns away from the source need fewer phases than the layout's.
+template <int S>
+__device__ inline int bp_col_phases(const ConeArgs &g, const BpCol &C) {
+  // fminf drops a NaN (no ray enters the column): the cap then applies
+  const float want = fminf(1.05f * g.bp_rps1 / C.amin, 1.0e6f);
+  const int w = max(1, (int)ceilf(want));
+  return min(g.bp_phases, (w + S - 1) / S);
+}
 
 // One block of the phased form: lane L holds rows vb + S*L + t, t < S;
 // every record's S rows are computed and written slot by slot into the one
@@ -881,8 +900,8 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
   const int min_v = C.min_v, max_v = C.max_v;
   const bool one_set = (min_v <= g.mid) && (max_v >= g.mid);
   const int npass = one_set ? 1 : 4;
-  // lane phases of the phased form (uniform count)
-  const int nph = PHASED ? g.bp_phases : 1;
+  // lane phases of the phased form (the column's; uniform in the wave)
+  const int nph = PHASED ? bp_col_phases<S>(g, C) : 1;
   const int ph = PHASED ? lane % nph : 0;
   constexpr int span = 64 * S;
   const uint32_t lane_off = 4u * (uint32_t)(S * lane);
@@ -1132,7 +1151,8 @@ BpLayout cone_bp_layout(const Geometry &G) {
   // a source close to the corner circle gives a tiny amin and a bound
   // past 64 * 16 rows: the phased form then runs one lane per phase,
   // which keeps every round's cells distinct whatever the rows per slice
-  const double rows_per_slice = G.vsize[2] / (amin * pitch);
+  L.rps1 = G.vsize[2] / pitch;
+  const double rows_per_slice = L.rps1 / amin;
   const int want = bp_rows_bound(rows_per_slice);
   L.stride = std::min(16, want);
   // lanes S rows apart are >= 1.05 slices apart: distinct slices
@@ -1154,16 +1174,23 @@ BpLayout cone_bp_layout(const Geometry &G) {
           : 1;
   L.phases = 1;
   if (L.phased) {
-    // 4, 8 or 16 slots; lanes of one phase are phases * S >= want rows
+    // 4, 8, 12 or 16 slots; lanes of one phase are phases * S >= want rows
     // apart.  Past 4 rows per slice, the S of least (padded rows) x (1 +
-    // 0.11 per extra phase): measured on wide cones, 768 rows: S = 16
-    // 681 ms, S = 8 in 2 phases 756 ms; 384 rows: 83.6 and 47.9 ms.
+    // 0.11 per extra phase), the smaller S on a tie: measured on wide
+    // cones, 768 rows: S = 16 681 ms, S = 8 in 2 phases 756 ms; 384 rows:
+    // 83.6 and 47.9 ms.  These are the layout's phases, for the column
+    // nearest the source; each column runs its own (bp_col_phases).
     auto phases = [&](int S) { return std::min(64, (want + S - 1) / S); };
     auto cost = [&](int S) {
       const double rows = (double)((nv + 64 * S - 1) / (64 * S)) * 64.0 * S;
       return rows * (1.0 + 0.11 * (phases(S) - 1));
     };
-    L.stride = want <= 4 ? 4 : (cost(8) <= cost(16) ? 8 : 16);
+    L.stride = 4;
+    if (want > 4) {
+      L.stride = 8;
+      for (int S : {12, 16})
+        if (cost(S) < cost(L.stride)) L.stride = S;
+    }
     L.phases = phases(L.stride);
   }
   CCPI_REQUIRE(bp_lds_bytes(nz, L.phased ? 1 : L.stride) <= 160 * 1024, CCPI_E_LIMIT,
@@ -1268,6 +1295,7 @@ void cone_build_plan(ConePlan &P, hipStream_t st) {
     A.bp_main_safe = L.main_safe;
     A.bp_phased = L.phased;
     A.bp_phases = L.phases;
+    A.bp_rps1 = (float)L.rps1;
     A.nvp = L.nvp;
   }
   // Walk parameter past which a row's ray is outside the volume's z range
@@ -1422,6 +1450,9 @@ static void cone_bp_launch(const ConePlan &P, const float *dpix, float *vol, hip
                        vol);
   else if (A.bp_phased && A.bp_stride == 8)
     hipLaunchKernelGGL((cone_bp_kernel<8, true, AccT>), grid, block, lds, st, A, bc, br, dpix,
+                       vol);
+  else if (A.bp_phased && A.bp_stride == 12)
+    hipLaunchKernelGGL((cone_bp_kernel<12, true, AccT>), grid, block, lds, st, A, bc, br, dpix,
                        vol);
   else if (A.bp_phased)
     hipLaunchKernelGGL((cone_bp_kernel<16, true, AccT>), grid, block, lds, st, A, bc, br, dpix,

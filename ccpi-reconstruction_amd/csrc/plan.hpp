@@ -143,9 +143,11 @@ long long plan_cache_bytes();
 
 // Accumulation layout of the cone BP for a geometry (host only): row stride
 // S, whether it is the phased one-array form and its lane phases, the bound
-// on detector rows per slice it was chosen for, and the padded row length.
+// on detector rows per slice it was chosen for, the padded row length, and
+// the rows per slice at walk parameter 1 (voxel z size / smallest pitch).
 struct BpLayout {
   int stride = 1, unique = 1, main_safe = 0, phased = 0, phases = 1, nvp = 0, want = 1;
+  double rps1 = 1.0;
 };
 BpLayout cone_bp_layout(const Geometry &g);
 

@@ -25,12 +25,16 @@ CASES = {
     "phased_s8_main": (CB((12, 12, 200), (1, 1, 1), A12, centred(20, 1.6), centred(460, 0.5),
                           20.0, 40.0), (8, True, 1)),
     "phased_s8_blocks": (CB((12, 12, 160), (1, 1, 0.25), A12, centred(20, 1.6),
-                            centred(700, 0.17), 20.0, 40.0), (8, True, 1)),
-    "phased_s16": (CB((12, 12, 40), (1, 1, 1), A12, centred(20, 1.6), centred(700, 0.25),
+                            centred(900, 0.17), 20.0, 40.0), (8, True, 1)),
+    "phased_s12": (CB((12, 12, 40), (1, 1, 1), A12, centred(20, 1.6), centred(700, 0.25),
+                      32.0, 64.0), (12, True, 1)),
+    "phased_s12_2ph": (CB((12, 12, 40), (1, 1, 1), A12, centred(20, 1.6), centred(700, 0.25),
+                          24.0, 48.0), (12, True, 2)),
+    "phased_s16": (CB((12, 12, 40), (1, 1, 1), A12, centred(20, 1.6), centred(1000, 0.25),
                       24.0, 48.0), (16, True, 1)),
     "phased_s8_2ph": (CB((12, 12, 40), (1, 1, 1), A12, centred(20, 1.6), centred(300, 0.25),
                          24.0, 48.0), (8, True, 2)),
-    "phased_s16_2ph": (CB((12, 12, 40), (1, 1, 1), A12, centred(20, 1.6), centred(700, 0.2),
+    "phased_s16_2ph": (CB((12, 12, 40), (1, 1, 1), A12, centred(20, 1.6), centred(1000, 0.2),
                           24.0, 48.0), (16, True, 2)),
     "phased_s4_lds": (CB((4, 4, 1500), (1, 1, 0.05), np.linspace(0, 2 * np.pi, 8, endpoint=False),
                          centred(10, 1.2), centred(3000, 0.05), 40.0, 80.0), (4, True, 1)),
@@ -40,7 +44,7 @@ CASES = {
     # 528 rows apart, on 640 rows that all fall within two slices there
     "phased_corner": (CB((8, 8, 6), (1, 1, 0.5), np.deg2rad([0, 45, 100, 135, 225, 260, 315]),
                          centred(14, 1.0), centred(640, 1.0), math.sqrt(32) + 0.002,
-                         2 * (math.sqrt(32) + 0.002)), (8, True, 64)),
+                         2 * (math.sqrt(32) + 0.002)), (12, True, 64)),
 }
 
 
@@ -74,7 +78,12 @@ def bp_path(g):
     def cost(s):
         return -(-nv // (64 * s)) * 64 * s * (1.0 + 0.11 * (phases(s) - 1))
 
-    s = 4 if want <= 4 else (8 if cost(8) <= cost(16) else 16)
+    s = 4
+    if want > 4:
+        s = 8
+        for c in (12, 16):          # the smaller S on a tie
+            if cost(c) < cost(s):
+                s = c
     return s, True, phases(s)
 
 

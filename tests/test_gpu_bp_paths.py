@@ -3,18 +3,22 @@
 The cone BP picks its row stride S from the largest number of detector rows
 per slice (at least 1.05 slices between the rows of one slot) and uses S
 LDS arrays per wavefront while S <= 4 and they fit 64 KB per workgroup.
-Otherwise it takes the phased form: S = 4, 8 or 16 slots on one array,
+Otherwise it takes the phased form: S = 4, 8, 12 or 16 slots on one array,
 written slot after slot, and within a slot the lanes in ceil(want / S)
-phases (past 4 rows per slice, S = 8 or 16 by the least padded rows x
-(1 + 0.11 per extra phase)).  These geometries select S = 1, 2 (the
-default test cases), 3, 4, and the phased form with:
+phases (past 4 rows per slice, S = 8, 12 or 16 by the least padded rows x
+(1 + 0.11 per extra phase)); each column runs its own phases, from the
+smallest walk parameter of its rays, at most the layout's.  These
+geometries select S = 1, 2 (the default test cases), 3, 4, and the phased
+form with:
   * S = 8: fewer than 64 rows, with the source at mid-height and near the
     volume's top; a padded block of main rows (every ray inside the
     volume's z range); two blocks, the second padded, with rows outside it;
-  * S = 8 in two phases (300 rows), S = 16 in one and in two phases (wide
-    cones: the volume's corners close to the source);
+  * S = 8 in two phases (300 rows); S = 12 in one and in two phases (700
+    rows: one block of 768), S = 16 in one and in two phases (1000 rows);
+    the wide cones' corners close to the source, so columns near the axis
+    run fewer phases than the corners;
   * S = 4 (arrays beyond the LDS budget);
-  * S = 8 in 64 phases: the source 0.002 mm outside the volume's corner
+  * S = 12 in 64 phases: the source 0.002 mm outside the volume's corner
     circle, where thousands of rows fall on one slice near the corners.
 Each is checked for FP and BP parity (1e-6 / 1e-5 of the maximum, as in
 test_gpu_parity) and the adjoint identity.  ``_path`` restates the host's
