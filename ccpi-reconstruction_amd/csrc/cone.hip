@@ -718,11 +718,16 @@ __device__ inline BpSlot bp_slot(const ConeArgs &g, int v, int pass, bool one_se
 // most the layout's g.bp_phases.  (LDS float atomics on one array made the
 // kernel twice as slow as these rounds, and rows interleaved over passes
 // re-read each record's detector row once per pass.)
-// At most 64 VGPRs (8 waves per SIMD) for strides 1 and 2: the only
-// spills this forces are in the per-row-block set-up, and the extra waves
-// hide the LDS read-add-write latency (S = 2 at 512^3: 98 -> 93 ms).  The
-// wider strides keep the registers their slots need.
-constexpr int bp_min_waves(int S) { return S <= 2 ? 8 : 1; }
+// Waves per SIMD the register allocation must allow: the extra waves hide
+// the LDS read-add-write latency, and the few spills they force sit in the
+// per-row-block set-up.  Strides 1 and 2: 8 (S = 2 at 512^3: 98 -> 93 ms).
+// Phased, measured on wide cones (ms per BP, waves per SIMD): S = 8 at
+// 384^3 32.2 (6) / 29.8 (7) / 29.1 (8); S = 12 at 768^3 442.6 (5) / 403.4 (6)
+// / 422.7 (7) / 602.0 (8); S = 16 at 1000^3 576.2 (4) / 503.0 (5) / 868.0
+// (6).  Strides 3 and 4 keep the registers their slots need.
+constexpr int bp_min_waves(int S) {
+  return S <= 2 ? 8 : (S == 8 ? 8 : (S == 12 ? 6 : (S == 16 ? 5 : 1)));
+}
 
 // Lane phases of the phased form for one column.  Its rays enter it at walk
 // parameter >= C.amin, where adjacent rows are amin * pitch apart in z, so
