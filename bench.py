@@ -46,16 +46,22 @@ for p in (os.path.join(ROOT, "ccpi-reconstruction_amd"), os.path.join(ROOT, "ora
 
 HBM_PEAK_GBS = 8000.0
 KERNEL_SOURCES = ("cone.hip", "parallel.hip", "par_body.hpp", "common.hpp")
+# the sources each counter summary's kernels are built from
+SUMMARY_SOURCES = {
+    "cone512_projectors_counters.md": ("cone.hip", "common.hpp"),
+    "parallel_256x256x16_cgls.md": ("parallel.hip", "par_body.hpp", "common.hpp"),
+}
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def source_stamp() -> str:
-    """SHA-256 (first 16 hex digits) of the projector kernel sources."""
+def source_stamp(names=KERNEL_SOURCES) -> str:
+    """SHA-256 (first 16 hex digits) of projector kernel sources (by default
+    all of them)."""
     h = hashlib.sha256()
-    for name in KERNEL_SOURCES:
+    for name in names:
         with open(os.path.join(ROOT, "ccpi-reconstruction_amd", "csrc", name), "rb") as f:
             h.update(f.read())
     return h.hexdigest()[:16]
@@ -80,7 +86,7 @@ def measured_traffic(kernel: str, summary: str = "cone512_projectors_counters.md
         return None, "missing"
     if val is None:
         return None, "missing"
-    if stamp != source_stamp():
+    if stamp != source_stamp(SUMMARY_SOURCES.get(summary, KERNEL_SOURCES)):
         return None, "stale"
     return val, "current"
 
