@@ -185,6 +185,36 @@ __global__ void cgls_update_xr_kernel(float *__restrict__ x, const float *__rest
   for (size_t q = t0; q < npix; q += stride) r[q] = r[q] + (-a) * Ad[q];
 }
 
+// The same update for the cone beam, fused with the BP's input: x += alpha
+// d, r -= alpha Ad, and dpix = r * d_conv in the BP's padded row layout
+// (the reference's scaled copy, c2D.cpp:1105-1114; cone_scale_kernel's
+// float product), so the BP of the new residual needs no scaling pass.
+// Grid-stride over the dpix elements (nrays * nvp) and the voxels.
+__global__ void cgls_update_xr_scale_kernel(float *__restrict__ x, const float *__restrict__ d,
+                                            size_t nvox, float *__restrict__ r,
+                                            const float *__restrict__ Ad, size_t nrays, int nh,
+                                            int nv, int nvp, const float *__restrict__ dconv,
+                                            float *__restrict__ dpix,
+                                            const CglsScalars *__restrict__ sc) {
+  const float a = sc->alpha_f;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (size_t q = t0; q < nvox; q += stride) x[q] = x[q] + a * d[q];
+  const size_t total = nrays * (size_t)nvp;
+  for (size_t q = t0; q < total; q += stride) {
+    const size_t ray = q / (size_t)nvp;
+    const int v = (int)(q - ray * (size_t)nvp);
+    float out = 0.0f;
+    if (v < nv) {
+      const size_t i = ray * (size_t)nv + v;
+      const float rn = r[i] + (-a) * Ad[i];
+      r[i] = rn;
+      out = rn * dconv[(size_t)(ray % (size_t)nh) * nv + v];
+    }
+    dpix[q] = out;
+  }
+}
+
 // d = s + beta d (scal_xby, blas.hpp:362-376)
 __global__ void cgls_update_d_kernel(float *__restrict__ d, const float *__restrict__ s,
                                      size_t n, const CglsScalars *__restrict__ sc) {
@@ -340,12 +370,26 @@ void CglsRun::iterate(int it, hipStream_t st) {
     hipLaunchKernelGGL(cgls_alpha_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
                        W.part.as<double>(), sc, 0);
   }
-  hipLaunchKernelGGL(cgls_update_xr_kernel, dim3(grid_for(std::max(nvox, npix))), dim3(256),
-                     0, st, x, d, nvox, b, Ad, npix, sc);
-  dev_fill(s, nvox, 0.f, st);
-  if (log) log->mark(log->bp, st);
-  P.bp(b, s, st, W.bp_scratch());
-  if (log) log->mark(log->bp, st);
+  if (P.cone) {
+    // the residual update writes the cone BP's scaled input as well
+    const ConeArgs &A = P.cone->args;
+    const size_t nrays = (size_t)A.na * A.nh;
+    float *dpix = W.bp_scratch();
+    hipLaunchKernelGGL(cgls_update_xr_scale_kernel,
+                       dim3(grid_for(std::max(nvox, nrays * (size_t)A.nvp))), dim3(256), 0, st,
+                       x, d, nvox, b, Ad, nrays, A.nh, A.nv, A.nvp, A.dconv, dpix, sc);
+    dev_fill(s, nvox, 0.f, st);
+    if (log) log->mark(log->bp, st);
+    P.bp_scaled(dpix, s, st);
+    if (log) log->mark(log->bp, st);
+  } else {
+    hipLaunchKernelGGL(cgls_update_xr_kernel, dim3(grid_for(std::max(nvox, npix))), dim3(256),
+                       0, st, x, d, nvox, b, Ad, npix, sc);
+    dev_fill(s, nvox, 0.f, st);
+    if (log) log->mark(log->bp, st);
+    P.bp(b, s, st, W.bp_scratch());
+    if (log) log->mark(log->bp, st);
+  }
   if (fp32) {
     sumsq<float>(s, nvox, W.part.p, st);
     hipLaunchKernelGGL(cgls_beta_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,

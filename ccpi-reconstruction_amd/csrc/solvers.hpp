@@ -27,7 +27,7 @@ struct Workspace {
   DeviceBuffer dscale;       // cone BP input scaled by d_conv (padded rows)
   ~Workspace();
   void ensure(const Projector &P);
-  // the workspace's BP scratch (nullptr for the parallel beam)
+  // the workspace's BP scratch (nullptr when the projector needs none)
   float *bp_scratch() const { return dscale.p ? dscale.as<float>() : nullptr; }
 };
 
