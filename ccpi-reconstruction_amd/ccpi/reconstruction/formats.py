@@ -8,8 +8,10 @@
     ``.bgs`` float dump (Core/src/results.cpp:23-228, Readers/tiff.cpp:
     118-176).
 
-TIFF is read and written here without libtiff: baseline strips,
-uncompressed, either byte order.  Host-side numpy throughout; the arrays
+TIFF is read and written here without libtiff: baseline strips, either
+byte order; the reader decodes uncompressed, LZW, Deflate and PackBits
+strips (with the horizontal predictor), the writer writes uncompressed
+strips.  Host-side numpy throughout; the arrays
 feed the device path through the ``conebeam`` / ``parallelbeam`` calls.
 """
 
@@ -19,6 +21,7 @@ import math
 import os
 import re
 import struct
+import zlib
 
 import numpy as np
 
@@ -63,11 +66,89 @@ def _tiff_tags(buf: bytes):
     return bo, tags
 
 
+def _unpackbits(raw: bytes) -> bytes:
+    """PackBits (TIFF 6.0 section 9): a header byte n, then n + 1 literal
+    bytes (0 <= n <= 127), or one byte repeated 1 - n times (-127 <= n <= -1);
+    n = -128 is a no-op."""
+    out = bytearray()
+    i, n = 0, len(raw)
+    while i < n:
+        h = raw[i]
+        i += 1
+        if h < 128:
+            out += raw[i:i + h + 1]
+            i += h + 1
+        elif h > 128:
+            if i < n:
+                out += bytes([raw[i]]) * (257 - h)
+            i += 1
+    return bytes(out)
+
+
+def _unlzw(raw: bytes) -> bytes:
+    """TIFF LZW (TIFF 6.0 section 13): MSB-first codes of 9 to 12 bits,
+    256 clears the table, 257 ends the strip; the code width grows one code
+    early (at table sizes 511, 1023, 2047)."""
+    table = [bytes([q]) for q in range(256)] + [b"", b""]
+    out = bytearray()
+    bits = 0
+    nbits = 0
+    width = 9
+    prev = None
+    pos = 0
+    n = len(raw)
+    while True:
+        while nbits < width and pos < n:
+            bits = (bits << 8) | raw[pos]
+            pos += 1
+            nbits += 8
+        if nbits < width:
+            break
+        code = (bits >> (nbits - width)) & ((1 << width) - 1)
+        nbits -= width
+        bits &= (1 << nbits) - 1
+        if code == 257:
+            break
+        if code == 256:
+            table = table[:258]
+            width = 9
+            prev = None
+            continue
+        if prev is None:
+            entry = table[code]
+        elif code < len(table):
+            entry = table[code]
+            table.append(prev + entry[:1])
+        elif code == len(table):
+            entry = prev + prev[:1]
+            table.append(entry)
+        else:
+            raise FormatError("corrupt LZW strip")
+        out += entry
+        prev = entry
+        if len(table) + 1 >= (1 << width) and width < 12:
+            width += 1
+    return bytes(out)
+
+
+def _decode_strip(raw: bytes, compression: int) -> bytes:
+    if compression == 1:
+        return raw
+    if compression == 5:
+        return _unlzw(raw)
+    if compression in (8, 32946):
+        return zlib.decompress(raw)
+    if compression == 32773:
+        return _unpackbits(raw)
+    raise FormatError("TIFF compression %d is not supported" % compression)
+
+
 def read_tiff16(path: str) -> np.ndarray:
     """One 16-bit grey-level TIFF as uint16 [rows][width], with the checks
     of read_tiff (tiff.cpp:22-116): strips (not tiles), 16 bits per sample,
-    min-is-black, MSB-to-LSB fill order.  Only uncompressed strips are
-    decoded."""
+    min-is-black, MSB-to-LSB fill order.  Strips may be uncompressed, LZW,
+    Deflate or PackBits, with or without the horizontal predictor (the
+    reference decodes them through libtiff)."""
     with open(path, "rb") as f:
         buf = f.read()
     bo, t = _tiff_tags(buf)
@@ -81,18 +162,25 @@ def read_tiff16(path: str) -> np.ndarray:
         raise FormatError("TIFF photometric type not supported by XTek reader")
     if t.get(266, (1,))[0] != 1:
         raise FormatError("TIFF is not MSB to LSB")
-    if t.get(259, (1,))[0] != 1:
-        raise FormatError("compressed TIFF strips are not supported")
     if t.get(277, (1,))[0] != 1:
         raise FormatError("TIFF has more than one sample per pixel")
+    compression = t.get(259, (1,))[0]
+    predictor = t.get(317, (1,))[0]
+    if predictor not in (1, 2):
+        raise FormatError("TIFF predictor %d is not supported" % predictor)
     width, rows = t[256][0], t[257][0]
-    data = b"".join(buf[o:o + c] for o, c in zip(t[273], t[279]))
+    data = b"".join(_decode_strip(buf[o:o + c], compression) for o, c in zip(t[273], t[279]))
     if len(data) % 2:
         raise FormatError("Odd number of bytes for 16 bit image")
     img = np.frombuffer(data, dtype=np.dtype(bo + "u2"))
     if img.size != width * rows:
         raise FormatError("Image size mismatch")
-    return img.reshape(rows, width).astype(np.uint16)
+    img = img.reshape(rows, width).astype(np.uint16)
+    if predictor == 2:
+        # horizontal differencing: each sample is stored minus its left
+        # neighbour, modulo 2^16
+        img = np.cumsum(img, axis=1, dtype=np.uint16)
+    return img
 
 
 def write_tiff(path: str, data: np.ndarray) -> None:

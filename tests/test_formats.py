@@ -1,8 +1,10 @@
 """Scan readers and result writers (CPU tests).
 
 TIFF: the reader is checked against files written by Pillow (an
-independent TIFF implementation) in both byte orders, and the writer by
-reading its files back with Pillow.  The XTek reader runs on synthetic
+independent TIFF implementation) in both byte orders, uncompressed and
+with LZW, PackBits and Deflate strips (and LZW with the horizontal
+predictor), its PackBits decoder against the TIFF 6.0 specification's
+example, and the writer by reading its files back with Pillow.  The XTek reader runs on synthetic
 scans written here (.xtekct, _ctdata.txt or .ang, 16-bit TIFFs from
 Pillow): the layout ([angle][h][v], v flipped from image rows and centred
 in the aligned rows), the angle lists (the .ang file in reverse), the
@@ -36,6 +38,40 @@ def test_read_tiff16_matches_pillow(tmp_path, big):
     p = str(tmp_path / "a.tif")
     _pil16(p, a, big)
     assert np.array_equal(F.read_tiff16(p), a)
+
+
+@pytest.mark.parametrize("comp", ["tiff_lzw", "packbits", "tiff_adobe_deflate"])
+@pytest.mark.parametrize("big", [False, True])
+def test_read_tiff16_compressed_strips(tmp_path, comp, big):
+    """LZW (table clears and 12-bit codes on noise), PackBits and Deflate
+    strips as Pillow's libtiff writes them, in both byte orders."""
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 65536, (257, 300)).astype(np.uint16)
+    a[100:140, :] = 4321                     # long runs
+    a[:, 7] = np.arange(257) * 255
+    mode, dt = ("I;16B", ">u2") if big else ("I;16", "<u2")
+    p = str(tmp_path / "c.tif")
+    Image.frombytes(mode, (a.shape[1], a.shape[0]), a.astype(dt).tobytes()).save(
+        p, compression=comp)
+    assert np.array_equal(F.read_tiff16(p), a)
+
+
+def test_read_tiff16_lzw_with_horizontal_predictor(tmp_path):
+    a = (np.add.outer(np.arange(64) * 3, np.arange(80) * 700) % 65536).astype(np.uint16)
+    p = str(tmp_path / "p.tif")
+    Image.frombytes("I;16", (80, 64), a.astype("<u2").tobytes()).save(
+        p, compression="tiff_lzw", tiffinfo={317: 2})
+    with open(p, "rb") as f:
+        _, tags = F._tiff_tags(f.read())
+    assert tags[259] == (5,) and tags[317] == (2,)
+    assert np.array_equal(F.read_tiff16(p), a)
+
+
+def test_unpackbits_known_answer():
+    """The PackBits example of the TIFF 6.0 specification (section 9)."""
+    packed = bytes.fromhex("FE AA 02 80 00 2A FD AA 03 80 00 2A 22 F7 AA")
+    want = bytes.fromhex("AA AA AA 80 00 2A AA AA AA AA 80 00 2A 22 AA AA AA AA AA AA AA AA AA AA")
+    assert F._unpackbits(packed) == want
 
 
 def test_read_tiff16_rejects_8bit(tmp_path):
