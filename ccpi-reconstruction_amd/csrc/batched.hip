@@ -3,8 +3,8 @@
 //
 // No counterpart in the reference (it reconstructs one problem per call;
 // the iteration body is cgls_3d, Core/src/Algorithms/cgls.cpp:26-157).
-// Every step of an iteration is ONE persistent launch over a flat work list
-// that spans all problems:
+// Every step of an iteration is ONE launch over a flat work list that spans
+// all problems:
 //   FP    items (problem, 256/lr-ray block, 4*lr-row block) -> Ad, partial ||Ad||^2
 //   alpha one workgroup per problem: fixed-order sum of its partials
 //   x/r   items (problem, element chunk)               -> x += a d, r -= a Ad
@@ -17,9 +17,10 @@
 // problem descriptors in device memory, so a replay needs no host work.
 // The norms of the projector outputs are produced by the projector
 // epilogues (per work item, deterministic), so no separate reduction pass
-// re-reads Ad or s.  Rows-per-lane and the persistent grid size are picked
-// by timing candidates on the batch itself (autotune, cached per batch
-// shape).
+// re-reads Ad or s.  Rows-per-lane and, per projector, the grid (one
+// workgroup or wavefront per item, or a persistent grid of 2 / 4 / 8
+// workgroups per CU dealing the items out by a fixed stride) are picked by
+// timing candidates on the batch itself (autotune, cached per batch shape).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -211,9 +212,15 @@ __global__ __launch_bounds__(256) void batch_vec_kernel(BatchProb *__restrict__ 
 
 namespace {
 
+// Launch shape of a batch: slices per BP item / rows per FP block (rc) and
+// the FP and BP grids: 0 launches one wavefront per FP item and one
+// workgroup per BP item (the dispatcher balances the uneven items), a
+// positive value is a persistent grid of that many workgroups (capped at
+// residency) that deals the items out by a fixed stride.
 struct Tuned {
   int rc;
-  int grid;
+  int fp_grid;
+  int bp_grid;
 };
 
 std::mutex g_tune_mu;
@@ -221,9 +228,9 @@ std::map<std::string, Tuned> g_tuned;
 
 int cu_count();
 
-// The persistent grid is capped at the workgroups that are resident at
-// once: the work items are dealt out by a fixed stride, so a workgroup that
-// only starts when another one ends would run its share at the tail.
+// A persistent grid is capped at the workgroups that are resident at once:
+// the work items are dealt out by a fixed stride, so a workgroup that only
+// starts when another one ends would run its share at the tail.
 template <class K>
 int blocks_per_cu(K kernel, size_t lds) {
   int per_cu = 0;
@@ -231,46 +238,50 @@ int blocks_per_cu(K kernel, size_t lds) {
   return std::max(per_cu, 1);
 }
 
-// lds: dynamic LDS of the BP launches (the tiled items' staged rows)
+// grid: Tuned's fp_grid / bp_grid; items: the launch's work items (FP
+// block items, four wavefront items each, or BP items); lds: dynamic LDS of
+// the BP launches (the tiled items' staged rows)
 template <int RC, int MODE, int BPK>
-void launch_one(int grid, size_t lds, BatchProb *P, const BatchLists &L, hipStream_t st) {
+void launch_one(int grid, int items, size_t lds, BatchProb *P, const BatchLists &L,
+                hipStream_t st) {
   const size_t l = MODE == 0 ? 0 : lds;
-  const int per_cu = blocks_per_cu(batch_proj_kernel<RC, MODE, BPK>, l);
-  const int g = std::max(1, std::min(grid, per_cu * cu_count()));
+  int g;
+  if (grid > 0) {
+    const int per_cu = blocks_per_cu(batch_proj_kernel<RC, MODE, BPK>, l);
+    g = std::min(grid, per_cu * cu_count());
+  } else {
+    g = MODE == 0 ? (items + 3) / 4 : items;
+  }
+  g = std::max(1, g);
   hipLaunchKernelGGL((batch_proj_kernel<RC, MODE, BPK>), dim3(g), dim3(256), l, st, P, L);
 }
 
 template <int RC, int MODE>
-void launch_bp(int bpk, int grid, size_t lds, BatchProb *P, const BatchLists &L,
+void launch_bp(int bpk, int grid, int items, size_t lds, BatchProb *P, const BatchLists &L,
                hipStream_t st) {
   if (bpk == 1)
-    launch_one<RC, MODE, 1>(grid, lds, P, L, st);
+    launch_one<RC, MODE, 1>(grid, items, lds, P, L, st);
   else if (bpk == 2)
-    launch_one<RC, MODE, 2>(grid, lds, P, L, st);
+    launch_one<RC, MODE, 2>(grid, items, lds, P, L, st);
   else
-    launch_one<RC, MODE, 3>(grid, lds, P, L, st);
+    launch_one<RC, MODE, 3>(grid, items, lds, P, L, st);
 }
 
 template <int RC>
-void launch_proj(int mode, int bpk, int grid, size_t lds, BatchProb *P, const BatchLists &L,
-                 hipStream_t st) {
+void launch_proj(int mode, int bpk, int grid, int items, size_t lds, BatchProb *P,
+                 const BatchLists &L, hipStream_t st) {
   if (mode == 0)
-    launch_one<RC, 0, 3>(grid, lds, P, L, st);  // the FP has no BP body
+    launch_one<RC, 0, 3>(grid, items, lds, P, L, st);  // the FP has no BP body
   else if (mode == 1)
-    launch_bp<RC, 1>(bpk, grid, lds, P, L, st);
+    launch_bp<RC, 1>(bpk, grid, items, lds, P, L, st);
   else
-    launch_bp<RC, 2>(bpk, grid, lds, P, L, st);
+    launch_bp<RC, 2>(bpk, grid, items, lds, P, L, st);
 }
 
-// bpk: BP bodies the batch needs (BatchRun::bp_kinds)
-void proj(int rc, int mode, int bpk, int grid, size_t lds, BatchProb *P, const BatchLists &L,
-          hipStream_t st) {
-  if (rc == 8)
-    launch_proj<8>(mode, bpk, grid, lds, P, L, st);
-  else
-    launch_proj<16>(mode, bpk, grid, lds, P, L, st);
-  CCPI_HIP(hipGetLastError());
-}
+struct BatchRun;
+// One projector step of every problem (mode 0 FP, 1 BP, 2 initial BP)
+// with the launch shape T.
+void proj(const BatchRun &R, const Tuned &T, int mode, hipStream_t st);
 
 struct BatchRun {
   std::vector<std::shared_ptr<ParPlan>> plans;
@@ -282,6 +293,17 @@ struct BatchRun {
   size_t bp_lds = 0;   // dynamic LDS of the BP launches
   int bp_kinds = 0;    // BP bodies of the problems: 1 tiled, 2 column-per-lane
 };
+
+void proj(const BatchRun &R, const Tuned &T, int mode, hipStream_t st) {
+  BatchProb *P = R.probs.as<BatchProb>();
+  const int grid = mode == 0 ? T.fp_grid : T.bp_grid;
+  const int items = mode == 0 ? R.fp_total : R.bp_total;
+  if (T.rc == 8)
+    launch_proj<8>(mode, R.bp_kinds, grid, items, R.bp_lds, P, R.L, st);
+  else
+    launch_proj<16>(mode, R.bp_kinds, grid, items, R.bp_lds, P, R.L, st);
+  CCPI_HIP(hipGetLastError());
+}
 
 // Item lists for row/slice blocks of `rc`.  A problem whose plan takes the
 // tiled BP gets tile items of rc slices, its staged span and angle groups
@@ -384,15 +406,16 @@ void run_pb_cgls_batched(const std::vector<Geometry> &geoms, float *const *b,
            "," + std::to_string(G.nx) + "," + std::to_string(G.ny) + "," + std::to_string(G.nz) + ";";
   }
 
-  // autotune rows-per-lane and the persistent grid once per batch shape
-  Tuned T{16, 4 * cu_count()};
+  // autotune rows-per-lane and the FP / BP grids once per batch shape
+  Tuned T{16, 0, 0};
   bool have = false;
   if (const char *env = std::getenv("CCPI_BATCH_TUNE")) {
-    // "rc,blocks_per_cu" pins the launch shape (skips the timing sweep)
-    int rc = 16, mult = 4;
-    if (std::sscanf(env, "%d,%d", &rc, &mult) == 2 && (rc == 8 || rc == 16) &&
-        mult > 0) {
-      T = Tuned{rc, mult * cu_count()};
+    // "rc,fp_blocks_per_cu,bp_blocks_per_cu" pins the launch shape (skips
+    // the timing sweep); 0 blocks per CU is one workgroup / wavefront per item
+    int rc = 16, fm = 0, bm = 0;
+    if (std::sscanf(env, "%d,%d,%d", &rc, &fm, &bm) == 3 && (rc == 8 || rc == 16) &&
+        fm >= 0 && bm >= 0) {
+      T = Tuned{rc, fm * cu_count(), bm * cu_count()};
       have = true;
     }
   }
@@ -405,32 +428,48 @@ void run_pb_cgls_batched(const std::vector<Geometry> &geoms, float *const *b,
     }
   }
   if (!have) {
-    hipEvent_t e0, e1;
+    hipEvent_t e0, e1, e2;
     CCPI_HIP(hipEventCreate(&e0));
     CCPI_HIP(hipEventCreate(&e1));
+    CCPI_HIP(hipEventCreate(&e2));
     float best = 1e30f;
     for (int rc : {8, 16}) {
       layout(R, rc, st);
-      for (int mult : {2, 4, 8}) {
-        const int grid = mult * cu_count();
-        // FP of d (scratch, contents irrelevant) and BP of r into s: the
-        // two projector launches of an iteration, outputs discarded
-        proj(rc, 0, R.bp_kinds, grid, R.bp_lds, R.probs.as<BatchProb>(), R.L, st);
+      // FP of d (scratch, contents irrelevant) and BP of r into s: the two
+      // projector launches of an iteration, outputs discarded; the first
+      // pair loads the kernels
+      Tuned cand{rc, 0, 0};
+      proj(R, cand, 0, st);
+      proj(R, cand, 1, st);
+      float fp_best = 1e30f, bp_best = 1e30f;
+      for (int mult : {0, 2, 4, 8}) {
+        const Tuned t{rc, mult * cu_count(), mult * cu_count()};
         CCPI_HIP(hipEventRecord(e0, st));
-        proj(rc, 0, R.bp_kinds, grid, R.bp_lds, R.probs.as<BatchProb>(), R.L, st);
-        proj(rc, 1, R.bp_kinds, grid, R.bp_lds, R.probs.as<BatchProb>(), R.L, st);
+        proj(R, t, 0, st);
         CCPI_HIP(hipEventRecord(e1, st));
-        CCPI_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        CCPI_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-          best = ms;
-          T = Tuned{rc, grid};
+        proj(R, t, 1, st);
+        CCPI_HIP(hipEventRecord(e2, st));
+        CCPI_HIP(hipEventSynchronize(e2));
+        float fp_ms = 0.f, bp_ms = 0.f;
+        CCPI_HIP(hipEventElapsedTime(&fp_ms, e0, e1));
+        CCPI_HIP(hipEventElapsedTime(&bp_ms, e1, e2));
+        if (fp_ms < fp_best) {
+          fp_best = fp_ms;
+          cand.fp_grid = t.fp_grid;
         }
+        if (bp_ms < bp_best) {
+          bp_best = bp_ms;
+          cand.bp_grid = t.bp_grid;
+        }
+      }
+      if (fp_best + bp_best < best) {
+        best = fp_best + bp_best;
+        T = cand;
       }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
     std::lock_guard<std::mutex> lock(g_tune_mu);
     g_tuned[sig] = T;
   }
@@ -439,7 +478,7 @@ void run_pb_cgls_batched(const std::vector<Geometry> &geoms, float *const *b,
   const int vgrid = 4 * cu_count();
 
   // start: d = A^T b, gamma = ||d||^2 (cgls.cpp:45-49)
-  proj(T.rc, 2, R.bp_kinds, T.grid, R.bp_lds, P, R.L, st);
+  proj(R, T, 2, st);
   hipLaunchKernelGGL(batch_scalar_kernel<0>, dim3(R.count), dim3(256), 0, st, P, R.L);
   CCPI_HIP(hipGetLastError());
 
@@ -447,10 +486,10 @@ void run_pb_cgls_batched(const std::vector<Geometry> &geoms, float *const *b,
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   CCPI_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  proj(T.rc, 0, R.bp_kinds, T.grid, R.bp_lds, P, R.L, st);
+  proj(R, T, 0, st);
   hipLaunchKernelGGL(batch_scalar_kernel<1>, dim3(R.count), dim3(256), 0, st, P, R.L);
   hipLaunchKernelGGL(batch_vec_kernel<0>, dim3(vgrid), dim3(256), 0, st, P, R.L);
-  proj(T.rc, 1, R.bp_kinds, T.grid, R.bp_lds, P, R.L, st);
+  proj(R, T, 1, st);
   hipLaunchKernelGGL(batch_scalar_kernel<2>, dim3(R.count), dim3(256), 0, st, P, R.L);
   hipLaunchKernelGGL(batch_vec_kernel<1>, dim3(vgrid), dim3(256), 0, st, P, R.L);
   CCPI_HIP(hipStreamEndCapture(st, &graph));
