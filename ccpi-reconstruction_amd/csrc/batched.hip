@@ -390,6 +390,10 @@ void run_pb_cgls_batched(const std::vector<Geometry> &geoms, float *const *b,
     R.plans.push_back(par_plan(G, st));
     BatchProb B{};
     B.g = R.plans.back()->args;
+    // the one launch holds every problem's rays: the FP walks them (a walk
+    // table's dependent load would lengthen the gathers' latency chain)
+    B.g.woff = nullptr;
+    B.g.went = nullptr;
     B.r = b[p];
     B.x = x[p];
     B.d = w;

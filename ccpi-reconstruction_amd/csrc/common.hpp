@@ -115,6 +115,11 @@ struct ParArgs {
   const int *map;        // [nv] row -> slice
   const int *rlo;        // [nz] first row of slice k
   const int *rhi;        // [nz] one past its last row
+  // walk table (null: the FP walks each ray itself): ray r's cells are
+  // entries woff[r] .. woff[r + 1] - 1 of went, {x * ny + y, float bits of
+  // the xy length}, in walk order
+  const unsigned long long *woff;
+  const uint2 *went;
 };
 
 }  // namespace ccpi

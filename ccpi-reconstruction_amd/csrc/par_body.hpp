@@ -28,6 +28,13 @@ __device__ inline float4 load_f4(const CCPI_GLOBAL float *p) {
   return v;
 }
 
+// One 8-byte walk-table entry.
+__device__ inline uint2 load_u2(const CCPI_GLOBAL uint2 *p) {
+  uint2 v;
+  __builtin_memcpy(&v, p, sizeof(v));
+  return v;
+}
+
 // The detector line of ray (a, h) walked through the xy grid (p2D.cpp:
 // 215-458): add(x, y, l) for every crossed cell, l its xy length.
 template <class Add>
@@ -114,9 +121,10 @@ __device__ inline void par_walk(const ParArgs &g, int a, int h, Add &&add) {
   }
 }
 
-// The walk of par_walk, G cells at a time: emit(cx, cy, l, n) receives the
-// next n <= G cells (n < G only for the last group) in the walk's order,
-// so a caller can issue the group's gathers before it accumulates them.
+// The walk of par_walk, G cells at a time: emit(cid, l, n) receives the
+// next n <= G cells (column index x * ny + y and xy length; n < G only for
+// the last group) in the walk's order, so a caller can issue the group's
+// gathers before it accumulates them.
 template <int G, class Emit>
 __device__ inline void par_walk_groups(const ParArgs &g, int a, int h, Emit &&emit) {
   const int nx = g.nx, ny = g.ny;
@@ -124,7 +132,7 @@ __device__ inline void par_walk_groups(const ParArgs &g, int a, int h, Emit &&em
   const double hp = gmem(g.h_pix)[h];
   const double p2x = c * g.D - s * hp;
   const double p2y = s * g.D + c * hp;
-  int cx[G], cy[G];
+  int cid[G];
   float l[G];
   if (fabs(c) < kEps) {
     if (fabs(s) >= kEps) {  // line parallel to y (p2D.cpp:233-260)
@@ -133,11 +141,10 @@ __device__ inline void par_walk_groups(const ParArgs &g, int a, int h, Emit &&em
         for (int j0 = 0; j0 < ny; j0 += G) {
 #pragma unroll
           for (int q = 0; q < G; q++) {
-            cx[q] = i;
-            cy[q] = s < 0.0 ? ny - 1 - (j0 + q) : j0 + q;
+            cid[q] = i * ny + (s < 0.0 ? ny - 1 - (j0 + q) : j0 + q);
             l[q] = (float)g.dy;
           }
-          emit(cx, cy, l, min(G, ny - j0));
+          emit(cid, l, min(G, ny - j0));
         }
       }
     }
@@ -147,11 +154,10 @@ __device__ inline void par_walk_groups(const ParArgs &g, int a, int h, Emit &&em
       for (int i0 = 0; i0 < nx; i0 += G) {
 #pragma unroll
         for (int q = 0; q < G; q++) {
-          cx[q] = c < 0.0 ? nx - 1 - (i0 + q) : i0 + q;
-          cy[q] = j;
+          cid[q] = (c < 0.0 ? nx - 1 - (i0 + q) : i0 + q) * ny + j;
           l[q] = (float)g.dx;
         }
-        emit(cx, cy, l, min(G, nx - i0));
+        emit(cid, l, min(G, nx - i0));
       }
     }
   } else {  // general line (p2D.cpp:284-458), steps as par_walk
@@ -193,8 +199,7 @@ __device__ inline void par_walk_groups(const ParArgs &g, int a, int h, Emit &&em
 #pragma unroll
         for (int q = 0; q < G; q++) {
           const bool in = x >= 0 && x < nx && y >= 0 && y < ny;
-          cx[q] = in ? x : 0;
-          cy[q] = in ? y : 0;
+          cid[q] = in ? x * ny + y : 0;
           l[q] = 0.f;
           if (in) {
             const double nxa = (g.ox + (double)(x + bxs) * g.dx - p2x) * ix;
@@ -216,9 +221,34 @@ __device__ inline void par_walk_groups(const ParArgs &g, int a, int h, Emit &&em
             n = q + 1;
           }
         }
-        emit(cx, cy, l, n);
+        emit(cid, l, n);
       }
     }
+  }
+}
+
+// The cells of ray r (angle a, column h), G at a time as par_walk_groups
+// emits them: read from the plan's walk table when it has one (the same
+// cells and lengths, written at plan time by par_walk_groups), else walked.
+template <int G, class Emit>
+__device__ inline void par_cells(const ParArgs &g, int r, int a, int h, Emit &&emit) {
+  if (g.woff) {
+    const unsigned long long e0 = gmem(g.woff)[r], e1 = gmem(g.woff)[r + 1];
+    int cid[G];
+    float l[G];
+    for (unsigned long long e = e0; e < e1; e += G) {
+      const int n = (int)min((unsigned long long)G, e1 - e);
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        uint2 t = make_uint2(0u, 0u);
+        if (q < n) t = load_u2(gmem(g.went) + e + q);
+        cid[q] = (int)t.x;
+        l[q] = __builtin_bit_cast(float, t.y);
+      }
+      emit(cid, l, n);
+    }
+  } else {
+    par_walk_groups<G>(g, a, h, emit);
   }
 }
 
@@ -236,7 +266,7 @@ __device__ inline double par_fp_rows(const ParArgs &g, int r, int v0,
                                      float *__restrict__ sino, bool acc_in) {
   static_assert(RC % 4 == 0, "rows are read in groups of four");
   const int a = r / g.nh, h = r - a * g.nh;
-  const int nz = g.nz, ny = g.ny;
+  const int nz = g.nz;
   CCPI_GLOBAL float *__restrict__ out = gmem(sino) + (size_t)r * g.nv + v0;
   const int nrow = min(RC, g.nv - v0);
   Acc acc[RC];
@@ -255,13 +285,11 @@ __device__ inline double par_fp_rows(const ParArgs &g, int r, int v0,
     // the accumulation, which adds the cells in the walk's order (the same
     // float operations as one cell at a time)
     constexpr int Q4 = RC / 4;
-    par_walk_groups<kParFpGroup>(g, a, h, [&](const int *cx, const int *cy, const float *l,
-                                              int n) {
+    par_cells<kParFpGroup>(g, r, a, h, [&](const int *cid, const float *l, int n) {
       float4 f[kParFpGroup][Q4];
 #pragma unroll
       for (int q = 0; q < kParFpGroup; q++) {
-        const CCPI_GLOBAL float *__restrict__ cv =
-            gmem(vol) + ((size_t)cx[q] * ny + cy[q]) * nz + v0;
+        const CCPI_GLOBAL float *__restrict__ cv = gmem(vol) + (size_t)cid[q] * nz + v0;
 #pragma unroll
         for (int q4 = 0; q4 < Q4; q4++) {
           float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -292,11 +320,13 @@ __device__ inline double par_fp_rows(const ParArgs &g, int r, int v0,
       }
     });
   } else {
-    par_walk(g, a, h, [&](int x, int y, float l) {
-      const CCPI_GLOBAL float *__restrict__ col = gmem(vol) + ((size_t)x * ny + y) * nz;
+    par_cells<kParFpGroup>(g, r, a, h, [&](const int *cid, const float *l, int n) {
+      for (int q = 0; q < n; q++) {
+        const CCPI_GLOBAL float *__restrict__ col = gmem(vol) + (size_t)cid[q] * nz;
 #pragma unroll
-      for (int q = 0; q < RC; q++)
-        if (kmap[q] >= 0) acc[q] = acc[q] + (Acc)(col[kmap[q]] * l);
+        for (int k = 0; k < RC; k++)
+          if (kmap[k] >= 0) acc[k] = acc[k] + (Acc)(col[kmap[k]] * l[q]);
+      }
     });
   }
   double ss = 0.0;

@@ -81,6 +81,33 @@ __global__ void par_bp_sum_kernel(const float *__restrict__ part, float *__restr
   }
 }
 
+// Walk table of the plan, pass 1: the number of cells of every ray (one
+// lane per ray, the FP's own walk).
+__global__ __launch_bounds__(256) void par_walk_count_kernel(ParArgs g,
+                                                             unsigned long long *__restrict__ cnt) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.na * g.nh) return;
+  const int a = r / g.nh, h = r - a * g.nh;
+  unsigned long long n = 0;
+  par_walk_groups<kParFpGroup>(g, a, h, [&](const int *, const float *, int m) { n += m; });
+  cnt[r] = n;
+}
+
+// Pass 2: the cells themselves, at the ray's offset.
+__global__ __launch_bounds__(256) void par_walk_fill_kernel(ParArgs g,
+                                                            const unsigned long long *__restrict__ off,
+                                                            uint2 *__restrict__ ent) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.na * g.nh) return;
+  const int a = r / g.nh, h = r - a * g.nh;
+  unsigned long long e = off[r];
+  par_walk_groups<kParFpGroup>(g, a, h, [&](const int *cid, const float *l, int m) {
+    for (int q = 0; q < m; q++)
+      ent[e + q] = make_uint2((unsigned)cid[q], __builtin_bit_cast(unsigned, l[q]));
+    e += m;
+  });
+}
+
 // Nonzero system-matrix entries: per ray, crossed cells with nonzero xy
 // length times the detector rows that map into the volume.
 __global__ __launch_bounds__(256) void par_nnz_kernel(ParArgs g,
@@ -257,7 +284,39 @@ void par_build_plan(ParPlan &P, hipStream_t st) {
   A.map = P.map.as<int>();
   A.rlo = P.rlo.as<int>();
   A.rhi = P.rhi.as<int>();
-  CCPI_HIP(hipStreamSynchronize(st));
+  // walk table: every ray's cells, so the FP reads them instead of walking
+  // the grid in double each time.  Only for FPs of fewer than
+  // kParWalkTableWaves wavefronts: those leave SIMDs idle and are bound by
+  // the walk's FP64 work (256 x 256 x 16, 180 angles: 0.182 -> 0.122 ms);
+  // larger ones are bound by their gathers' latency, which the table's
+  // dependent load lengthens (the 64-problem mix: 25.5 -> 27.0 ms).
+  const size_t rays = (size_t)G.na * G.nh;
+  if (par_walk_table_wanted(G)) {
+    P.woff.alloc((rays + 1) * sizeof(unsigned long long));
+    unsigned long long *off = P.woff.as<unsigned long long>();
+    const int blocks = (int)((rays + 255) / 256);
+    hipLaunchKernelGGL(par_walk_count_kernel, dim3(blocks), dim3(256), 0, st, A, off);
+    CCPI_HIP(hipGetLastError());
+    std::vector<unsigned long long> h(rays + 1);
+    CCPI_HIP(hipMemcpyAsync(h.data(), off, rays * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, st));
+    CCPI_HIP(hipStreamSynchronize(st));
+    unsigned long long total = 0;
+    for (size_t q = 0; q < rays; q++) {
+      const unsigned long long n = h[q];
+      h[q] = total;
+      total += n;
+    }
+    h[rays] = total;
+    P.woff.upload(h, st);
+    P.went.alloc(std::max<unsigned long long>(total, 1) * sizeof(uint2));
+    hipLaunchKernelGGL(par_walk_fill_kernel, dim3(blocks), dim3(256), 0, st, A,
+                       P.woff.as<unsigned long long>(), P.went.as<uint2>());
+    CCPI_HIP(hipGetLastError());
+    A.woff = P.woff.as<unsigned long long>();
+    A.went = P.went.as<uint2>();
+  }
+  CCPI_HIP(hipStreamSynchronize(st));  // h is the upload's source
   P.args = A;
   P.bp = par_bp_shape(G, A);
 }

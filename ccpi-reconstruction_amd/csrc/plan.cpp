@@ -179,7 +179,10 @@ size_t total_bytes(const Cache &c) {
 // about two detector columns per angle and voxel column.  Used to make room
 // before a build, so the cache's peak is not the old plans plus the new one.
 size_t plan_estimate(const Geometry &g) {
-  if (g.kind != CCPI_CONE) return (size_t)g.na * 48 + (size_t)(g.nh + g.nv + 2 * g.nz) * 8;
+  if (g.kind != CCPI_CONE) {
+    return (size_t)g.na * 48 + (size_t)(g.nh + g.nv + 2 * g.nz) * 8 +
+           (par_walk_table_wanted(g) ? par_walk_table_bound(g) : 0);
+  }
   const size_t rays = (size_t)g.na * g.nh, cols = (size_t)g.nx * g.ny;
   return rays * (48 + 12 * (size_t)(g.nx + g.ny)) + cols * (32 + 32 * (size_t)g.na);
 }

@@ -130,11 +130,30 @@ struct ParPlan {
   ParArgs args{};
   ParBpShape bp{};
   DeviceBuffer atab, hpix, map, rlo, rhi;
-  SharedScratch scratch;  // BP partials of callers without a workspace
+  DeviceBuffer woff, went;  // walk table (args.woff / args.went)
+  SharedScratch scratch;    // BP partials of callers without a workspace
   size_t bytes() const {
-    return atab.bytes + hpix.bytes + map.bytes + rlo.bytes + rhi.bytes + scratch.bytes();
+    return atab.bytes + hpix.bytes + map.bytes + rlo.bytes + rhi.bytes + woff.bytes +
+           went.bytes + scratch.bytes();
   }
 };
+
+// Walk table of a parallel plan: built for FPs of fewer than
+// kParWalkTableWaves wavefronts (4 detector rows per lane) when its bound
+// (8 bytes per cell, at most nx + ny cells per ray, plus the ray offsets)
+// is within kParWalkTableBytes.
+constexpr size_t kParWalkTableBytes = size_t(4) << 30;
+constexpr long long kParWalkTableWaves = 8192;
+inline size_t par_walk_table_bound(const Geometry &g) {
+  const size_t rays = (size_t)g.na * g.nh;
+  return rays * 8 * (size_t)(g.nx + g.ny) + (rays + 1) * 8;
+}
+inline long long par_fp_waves(const Geometry &g) {
+  return (long long)g.na * g.nh * ((g.nv + 3) / 4) / 64;
+}
+inline bool par_walk_table_wanted(const Geometry &g) {
+  return par_walk_table_bound(g) <= kParWalkTableBytes && par_fp_waves(g) < kParWalkTableWaves;
+}
 
 std::shared_ptr<ConePlan> cone_plan(const Geometry &g, hipStream_t st);
 std::shared_ptr<ParPlan> par_plan(const Geometry &g, hipStream_t st);

@@ -1284,7 +1284,6 @@ int FN(or_cgls_reg)(const or_geom *g, RT *b, RT *x, int niter, int kind,
   for (it = 0; it < niter; it++) {
     RT alpha;
     double nn, beta;
-    const double t_start = omp_get_wtime();
     memset(Ad, 0, sizeof(RT) * (size_t)npix);
     FN(fp)(g, d, Ad);
     if (kind == 0) FN(or_tikhonov)(d, L, g->nx, g->ny, g->nz);
