@@ -122,4 +122,28 @@ struct ParArgs {
   const uint2 *went;
 };
 
+// ---- projector epilogue norms ----------------------------------------------
+// A projector given a `part` array leaves FP64 partial sums of the squares
+// of the values it wrote there (a fixed number per launch, each from a fixed
+// set of threads in a fixed order), so a solver's reduction over `part` is
+// deterministic and no pass re-reads the projector's output for its norm.
+
+// Sum of v over the 64 lanes of a wavefront (every lane gets it).
+__device__ inline double wave_sum64(double v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// Partial of a 256-thread workgroup: the lanes' values summed per wavefront,
+// then the four wavefronts in order into part[idx].  Every thread of the
+// workgroup calls it (it synchronises).
+__device__ inline void block_part256(double v, double *part, size_t idx) {
+  __shared__ double sh[4];
+  v = wave_sum64(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) part[idx] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
 }  // namespace ccpi

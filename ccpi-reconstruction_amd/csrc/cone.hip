@@ -364,7 +364,8 @@ __device__ inline Acc fp_run(const FpRow &R, const ConeEnt *__restrict__ e,
 template <class Acc>
 __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
     ConeArgs g, const ConeRay *__restrict__ rays, const ConeEnt *__restrict__ ents,
-    const float *__restrict__ eai, const float *__restrict__ vol, float *__restrict__ sino) {
+    const float *__restrict__ eai, const float *__restrict__ vol, float *__restrict__ sino,
+    int acc_in, double *__restrict__ norm) {
   extern __shared__ float s_vz[];
   for (int q = threadIdx.x; q <= g.nz; q += kFpRows) s_vz[q] = g.vz[q];
   __syncthreads();
@@ -379,7 +380,7 @@ __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
   const int h = r % g.nh;
   const ConeRay R = rays[r];
   float *out = sino + (size_t)r * g.nv + v;
-  Acc acc = (Acc)*out;
+  Acc acc = acc_in ? (Acc)*out : (Acc)0;
   // A row whose ray has left the volume's z range at the start of a
   // segment adds nothing from there on (c2D.cpp:153-187 break): the wave
   // stops at the largest such walk parameter of its rows.
@@ -410,7 +411,10 @@ __global__ __launch_bounds__(kFpRows) void cone_fp_kernel(
     row.ks = edge ? 1 : 0;
     acc = fp_run<true, Acc>(row, e, ai, vol, nz, ncom, R.n, amax_bits, acc);
   }
-  if (live) *out = (float)(acc * (Acc)g.dconv[(size_t)h * g.nv + v]);
+  const float o = (float)(acc * (Acc)g.dconv[(size_t)h * g.nv + v]);
+  if (live) *out = o;
+  if (norm) block_part256(live ? (double)o * (double)o : 0.0, norm,
+                          (size_t)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // eai[q] = ents[q - 1].alpha / voxel z size: the slice-position factor of
@@ -869,7 +873,8 @@ template <int S, bool PHASED, class AccT>
 __global__ __launch_bounds__(64 * kBpWaves)
 __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
     ConeArgs g, const BpCol *__restrict__ cols, const BpRec *__restrict__ recs,
-    const float *__restrict__ dpix, float *__restrict__ vol) {
+    const float *__restrict__ dpix, float *__restrict__ vol, int acc_in,
+    double *__restrict__ norm) {
   constexpr int narr = PHASED ? 1 : S;  // LDS arrays per wavefront
   constexpr bool kFloatAcc = std::is_same<AccT, float>::value;
   extern __shared__ float smem[];
@@ -894,12 +899,27 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
   if (ci >= g.nx || cj >= g.ny) return;
   const int colid = ci * g.ny + cj;
   const BpCol C = cols[colid];
-  if (C.n == 0) return;  // nothing crosses the column: it stays as it is
-  AccT *acc = reinterpret_cast<AccT *>(smem + lds_pad4(nz + 2)) + wave * narr * astride;
   float *__restrict__ vcol = vol + (size_t)colid * nz;
+  if (C.n == 0) {
+    // nothing crosses the column: it stays as it is (zero when not
+    // accumulating); its norm partial is that of what it holds
+    double ss = 0.0;
+    for (int k = lane; k < nz; k += 64) {
+      const float o = acc_in ? vcol[k] : 0.0f;
+      if (!acc_in) vcol[k] = o;
+      ss += (double)o * (double)o;
+    }
+    if (norm) {
+      ss = wave_sum64(ss);
+      if (lane == 0) norm[colid] = ss;
+    }
+    return;
+  }
+  AccT *acc = reinterpret_cast<AccT *>(smem + lds_pad4(nz + 2)) + wave * narr * astride;
   for (int q = lane; q < narr * astride; q += 64) acc[q] = (AccT)0;
   __builtin_amdgcn_wave_barrier();
-  for (int k = lane; k < nz; k += 64) acc[k + 1] = vcol[k];
+  if (acc_in)
+    for (int k = lane; k < nz; k += 64) acc[k + 1] = vcol[k];
   __builtin_amdgcn_wave_barrier();
 
   const int min_v = C.min_v, max_v = C.max_v;
@@ -1089,11 +1109,18 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
   }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  double ss = 0.0;
   for (int k = lane; k < nz; k += 64) {
     AccT s = acc[k + 1];
 #pragma unroll
     for (int t = 1; t < narr; ++t) s += acc[t * astride + k + 1];
-    vcol[k] = (float)s;
+    const float o = (float)s;
+    vcol[k] = o;
+    ss += (double)o * (double)o;
+  }
+  if (norm) {
+    ss = wave_sum64(ss);
+    if (lane == 0) norm[colid] = ss;
   }
 }
 
@@ -1398,18 +1425,25 @@ void cone_build_plan(ConePlan &P, hipStream_t st) {
   P.args = A;
 }
 
-void cone_fp(const ConePlan &P, const float *vol, float *sino, hipStream_t st, bool f64) {
+size_t cone_fp_norm_parts(const ConePlan &P) {
+  const ConeArgs &A = P.args;
+  return (size_t)A.na * A.nh * (size_t)((A.nv + kFpRows - 1) / kFpRows);
+}
+
+void cone_fp(const ConePlan &P, const float *vol, float *sino, hipStream_t st, bool f64,
+             bool acc_in, double *norm) {
   const ConeArgs &A = P.args;
   dim3 grid((unsigned)(A.na * A.nh), (unsigned)((A.nv + kFpRows - 1) / kFpRows));
   const size_t lds = sizeof(float) * (size_t)(A.nz + 1);
+  const int ai = acc_in ? 1 : 0;
   if (f64)
     hipLaunchKernelGGL(cone_fp_kernel<double>, grid, dim3(kFpRows), lds, st, A,
                        P.rays.as<ConeRay>(), P.ents.as<ConeEnt>(), P.eai.as<float>(), vol,
-                       sino);
+                       sino, ai, norm);
   else
     hipLaunchKernelGGL(cone_fp_kernel<float>, grid, dim3(kFpRows), lds, st, A,
                        P.rays.as<ConeRay>(), P.ents.as<ConeEnt>(), P.eai.as<float>(), vol,
-                       sino);
+                       sino, ai, norm);
   CCPI_HIP(hipGetLastError());
 }
 
@@ -1436,8 +1470,12 @@ void cone_scale(const ConePlan &P, const float *pix, float *dpix, hipStream_t st
   CCPI_HIP(hipGetLastError());
 }
 
+size_t cone_bp_norm_parts(const ConePlan &P) { return (size_t)P.args.nx * P.args.ny; }
+
 template <class AccT>
-static void cone_bp_launch(const ConePlan &P, const float *dpix, float *vol, hipStream_t st) {
+static void cone_bp_launch(const ConePlan &P, const float *dpix, float *vol, hipStream_t st,
+                           bool acc_in, double *norm) {
+  const int ai = acc_in ? 1 : 0;
   const ConeArgs &A = P.args;
   // kBpTile x kBpTile column tiles of kBpTile^2 / kBpWaves workgroups
   // (kernel mapping)
@@ -1452,37 +1490,37 @@ static void cone_bp_launch(const ConePlan &P, const float *dpix, float *vol, hip
   const dim3 grid(blocks), block(64 * kBpWaves);
   if (A.bp_phased && A.bp_stride == 4)
     hipLaunchKernelGGL((cone_bp_kernel<4, true, AccT>), grid, block, lds, st, A, bc, br, dpix,
-                       vol);
+                       vol, ai, norm);
   else if (A.bp_phased && A.bp_stride == 8)
     hipLaunchKernelGGL((cone_bp_kernel<8, true, AccT>), grid, block, lds, st, A, bc, br, dpix,
-                       vol);
+                       vol, ai, norm);
   else if (A.bp_phased && A.bp_stride == 12)
     hipLaunchKernelGGL((cone_bp_kernel<12, true, AccT>), grid, block, lds, st, A, bc, br, dpix,
-                       vol);
+                       vol, ai, norm);
   else if (A.bp_phased)
     hipLaunchKernelGGL((cone_bp_kernel<16, true, AccT>), grid, block, lds, st, A, bc, br, dpix,
-                       vol);
+                       vol, ai, norm);
   else if (narr == 1)
     hipLaunchKernelGGL((cone_bp_kernel<1, false, AccT>), grid, block, lds, st, A, bc, br, dpix,
-                       vol);
+                       vol, ai, norm);
   else if (narr == 2)
     hipLaunchKernelGGL((cone_bp_kernel<2, false, AccT>), grid, block, lds, st, A, bc, br, dpix,
-                       vol);
+                       vol, ai, norm);
   else if (narr == 3)
     hipLaunchKernelGGL((cone_bp_kernel<3, false, AccT>), grid, block, lds, st, A, bc, br, dpix,
-                       vol);
+                       vol, ai, norm);
   else
     hipLaunchKernelGGL((cone_bp_kernel<4, false, AccT>), grid, block, lds, st, A, bc, br, dpix,
-                       vol);
+                       vol, ai, norm);
   CCPI_HIP(hipGetLastError());
 }
 
 void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, hipStream_t st,
-                    bool f64) {
+                    bool f64, bool acc_in, double *norm) {
   if (f64)
-    cone_bp_launch<double>(P, dpix, vol, st);
+    cone_bp_launch<double>(P, dpix, vol, st, acc_in, norm);
   else
-    cone_bp_launch<float>(P, dpix, vol, st);
+    cone_bp_launch<float>(P, dpix, vol, st, acc_in, norm);
 }
 
 }  // namespace ccpi

@@ -37,48 +37,70 @@ namespace ccpi {
 // 16 needs 121 (4 waves) and was 5 % slower on the 64-problem workload
 constexpr int kRows = 8;
 
-// Acc: float sums (the reference's) or double (FP64-accumulation option)
+// Acc: float sums (the reference's) or double (FP64-accumulation option).
+// acc_in: y += A x (the reference's f2D) or y = A x; norm: null, or one
+// partial of ||y||^2 per workgroup (epilogue norms, common.hpp).
 template <class Acc>
 __global__ __launch_bounds__(256) void par_fp_kernel(ParArgs g, int lr,
                                                      const float *__restrict__ vol,
-                                                     float *__restrict__ sino) {
-  par_fp_lanes<Acc>(g, blockIdx.x, blockIdx.y, threadIdx.x, lr, vol, sino, true);
+                                                     float *__restrict__ sino, int acc_in,
+                                                     double *__restrict__ norm) {
+  const double ss =
+      par_fp_lanes<Acc>(g, blockIdx.x, blockIdx.y, threadIdx.x, lr, vol, sino, acc_in != 0);
+  if (norm) block_part256(ss, norm, (size_t)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 template <class Acc>
 __global__ __launch_bounds__(256) void par_bp_kernel(ParArgs g,
                                                      const float *__restrict__ sino,
-                                                     float *__restrict__ vol) {
-  par_bp_thread<kRows, Acc>(g, blockIdx.x, blockIdx.y, threadIdx.x, sino, vol, true);
+                                                     float *__restrict__ vol, int acc_in,
+                                                     double *__restrict__ norm) {
+  const double ss = par_bp_thread<kRows, Acc>(g, blockIdx.x, blockIdx.y, threadIdx.x, sino,
+                                              vol, acc_in != 0);
+  if (norm) block_part256(ss, norm, (size_t)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // The tiled BP as one launch: blockIdx.x = tile, y = slice chunk, z = angle
-// group; one group adds into vol, several write their partial sums to
-// part[group] for par_bp_sum_kernel.
+// group; one group writes vol (adding to it when acc_in), several write
+// their partial sums to part[group] for par_bp_sum_kernel.  norm (one
+// group only): one partial of ||vol||^2 per workgroup.
 template <int R, class Acc>
 __global__ __launch_bounds__(256) void par_bp_lds_kernel(ParArgs g, int hmax, int ab, int groups,
                                                          const float *__restrict__ sino,
                                                          float *__restrict__ vol,
-                                                         float *__restrict__ part) {
+                                                         float *__restrict__ part, int acc_in,
+                                                         double *__restrict__ norm) {
   extern __shared__ float smem[];
   const int grp = blockIdx.z;
   const int a_begin = (int)((long long)g.na * grp / groups);
   const int a_end = (int)((long long)g.na * (grp + 1) / groups);
   float *dst = groups == 1 ? vol : part + (size_t)grp * g.nx * g.ny * g.nz;
-  (void)par_bp_tile<R, Acc>(g, blockIdx.x, blockIdx.y, a_begin, a_end, hmax, ab, groups == 1,
-                            sino, dst, smem);
+  const double ss = par_bp_tile<R, Acc>(g, blockIdx.x, blockIdx.y, a_begin, a_end, hmax, ab,
+                                        groups == 1 && acc_in != 0, sino, dst, smem);
+  if (norm) block_part256(ss, norm, (size_t)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
-// vol += sum of the angle groups' partials, in group order (in Acc).
+// Workgroups of par_bp_sum_kernel (a fixed grid: its norm partials are a
+// fixed set).
+constexpr int kParSumBlocks = 2048;
+
+// vol (+)= sum of the angle groups' partials, in group order (in Acc);
+// norm: one partial of ||vol||^2 per workgroup.
 template <class Acc>
-__global__ void par_bp_sum_kernel(const float *__restrict__ part, float *__restrict__ vol,
-                                  size_t n, int groups) {
+__global__ __launch_bounds__(256) void par_bp_sum_kernel(const float *__restrict__ part,
+                                                         float *__restrict__ vol, size_t n,
+                                                         int groups, int acc_in,
+                                                         double *__restrict__ norm) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
+  double ss = 0.0;
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
     Acc s = (Acc)part[q];
     for (int p = 1; p < groups; p++) s = s + (Acc)part[(size_t)p * n + q];
-    vol[q] = (float)((Acc)vol[q] + s);
+    const float o = acc_in ? (float)((Acc)vol[q] + s) : (float)s;
+    vol[q] = o;
+    ss += (double)o * (double)o;
   }
+  if (norm) block_part256(ss, norm, blockIdx.x);
 }
 
 // Walk table of the plan, pass 1: the number of cells of every ray (one
@@ -321,15 +343,31 @@ void par_build_plan(ParPlan &P, hipStream_t st) {
   P.bp = par_bp_shape(G, A);
 }
 
-void par_fp(const ParPlan &P, const float *vol, float *sino, hipStream_t st, bool f64) {
-  const ParArgs &A = P.args;
+static dim3 par_fp_grid(const ParArgs &A, int &lr) {
   const int rays = A.na * A.nh;
-  const int lr = par_fp_lanes_per_ray(A.nv), rpb = 256 / lr;
-  dim3 grid((unsigned)((rays + rpb - 1) / rpb), (unsigned)((A.nv + 4 * lr - 1) / (4 * lr)));
+  lr = par_fp_lanes_per_ray(A.nv);
+  const int rpb = 256 / lr;
+  return dim3((unsigned)((rays + rpb - 1) / rpb), (unsigned)((A.nv + 4 * lr - 1) / (4 * lr)));
+}
+
+size_t par_fp_norm_parts(const ParPlan &P) {
+  int lr;
+  const dim3 g = par_fp_grid(P.args, lr);
+  return (size_t)g.x * g.y;
+}
+
+void par_fp(const ParPlan &P, const float *vol, float *sino, hipStream_t st, bool f64,
+            bool acc_in, double *norm) {
+  const ParArgs &A = P.args;
+  int lr;
+  const dim3 grid = par_fp_grid(A, lr);
+  const int ai = acc_in ? 1 : 0;
   if (f64)
-    hipLaunchKernelGGL(par_fp_kernel<double>, grid, dim3(256), 0, st, A, lr, vol, sino);
+    hipLaunchKernelGGL(par_fp_kernel<double>, grid, dim3(256), 0, st, A, lr, vol, sino, ai,
+                       norm);
   else
-    hipLaunchKernelGGL(par_fp_kernel<float>, grid, dim3(256), 0, st, A, lr, vol, sino);
+    hipLaunchKernelGGL(par_fp_kernel<float>, grid, dim3(256), 0, st, A, lr, vol, sino, ai,
+                       norm);
   CCPI_HIP(hipGetLastError());
 }
 
@@ -337,44 +375,63 @@ size_t par_bp_scratch_floats(const ParPlan &P) {
   return P.bp.tiled && P.bp.groups > 1 ? (size_t)P.bp.groups * P.g.n_vox() : 0;
 }
 
-template <class Acc>
-static void par_bp_launch(const ParPlan &P, const float *sino, float *vol, float *scratch,
-                          hipStream_t st) {
+// Grids of the BP's launches: the column-per-lane kernel, or the tiled
+// kernel and (with angle groups) the group sum, which writes the volume.
+static dim3 par_bp_grid(const ParPlan &P) {
   const ParArgs &A = P.args;
   const ParBpShape &S = P.bp;
+  if (!S.tiled)
+    return dim3((unsigned)((A.nx * A.ny + 255) / 256), (unsigned)((A.nz + kRows - 1) / kRows));
+  const unsigned tiles = (unsigned)(((A.nx + 15) / 16) * ((A.ny + 15) / 16));
+  return dim3(tiles, (unsigned)((A.nz + S.slices - 1) / S.slices), (unsigned)S.groups);
+}
+
+static int par_bp_sum_blocks(const ParPlan &P) {
+  return (int)std::min<size_t>((P.g.n_vox() + 255) / 256, kParSumBlocks);
+}
+
+size_t par_bp_norm_parts(const ParPlan &P) {
+  if (P.bp.tiled && P.bp.groups > 1) return (size_t)par_bp_sum_blocks(P);
+  const dim3 g = par_bp_grid(P);
+  return (size_t)g.x * g.y;
+}
+
+template <class Acc>
+static void par_bp_launch(const ParPlan &P, const float *sino, float *vol, float *scratch,
+                          hipStream_t st, bool acc_in, double *norm) {
+  const ParArgs &A = P.args;
+  const ParBpShape &S = P.bp;
+  const dim3 grid = par_bp_grid(P);
+  const int ai = acc_in ? 1 : 0;
   if (!S.tiled) {
-    const int cols = A.nx * A.ny;
-    dim3 grid((unsigned)((cols + 255) / 256), (unsigned)((A.nz + kRows - 1) / kRows));
-    hipLaunchKernelGGL(par_bp_kernel<Acc>, grid, dim3(256), 0, st, A, sino, vol);
+    hipLaunchKernelGGL(par_bp_kernel<Acc>, grid, dim3(256), 0, st, A, sino, vol, ai, norm);
     CCPI_HIP(hipGetLastError());
     return;
   }
   CCPI_REQUIRE(S.groups == 1 || scratch, CCPI_E_ARG, "parallel BP: angle groups need scratch");
-  const unsigned tiles = (unsigned)(((A.nx + 15) / 16) * ((A.ny + 15) / 16));
-  const dim3 grid(tiles, (unsigned)((A.nz + S.slices - 1) / S.slices), (unsigned)S.groups);
   const size_t lds = par_bp_lds_bytes(S);
+  // with angle groups the sum kernel writes the volume (and its norm)
+  double *tile_norm = S.groups == 1 ? norm : nullptr;
   if (S.slices == 8)
     hipLaunchKernelGGL((par_bp_lds_kernel<8, Acc>), grid, dim3(256), lds, st, A, S.hmax, S.ab,
-                       S.groups, sino, vol, scratch);
+                       S.groups, sino, vol, scratch, ai, tile_norm);
   else
     hipLaunchKernelGGL((par_bp_lds_kernel<16, Acc>), grid, dim3(256), lds, st, A, S.hmax,
-                       S.ab, S.groups, sino, vol, scratch);
+                       S.ab, S.groups, sino, vol, scratch, ai, tile_norm);
   CCPI_HIP(hipGetLastError());
   if (S.groups > 1) {
-    const size_t n = P.g.n_vox();
-    const int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(par_bp_sum_kernel<Acc>, dim3(blocks), dim3(256), 0, st, scratch, vol,
-                       n, S.groups);
+    hipLaunchKernelGGL(par_bp_sum_kernel<Acc>, dim3(par_bp_sum_blocks(P)), dim3(256), 0, st,
+                       scratch, vol, P.g.n_vox(), S.groups, ai, norm);
     CCPI_HIP(hipGetLastError());
   }
 }
 
 void par_bp(const ParPlan &P, const float *sino, float *vol, float *scratch, hipStream_t st,
-            bool f64) {
+            bool f64, bool acc_in, double *norm) {
   if (f64)
-    par_bp_launch<double>(P, sino, vol, scratch, st);
+    par_bp_launch<double>(P, sino, vol, scratch, st, acc_in, norm);
   else
-    par_bp_launch<float>(P, sino, vol, scratch, st);
+    par_bp_launch<float>(P, sino, vol, scratch, st, acc_in, norm);
 }
 
 // ---------------------------------------------------------------------------

@@ -267,30 +267,40 @@ Projector::Projector(const Geometry &g, hipStream_t st) {
     par = par_plan(g, st);
 }
 
-void Projector::fp(const float *vol, float *sino, hipStream_t st, bool f64) const {
+void Projector::fp(const float *vol, float *sino, hipStream_t st, bool f64, bool acc_in,
+                   double *norm) const {
   if (cone)
-    cone_fp(*cone, vol, sino, st, f64);
+    cone_fp(*cone, vol, sino, st, f64, acc_in, norm);
   else
-    par_fp(*par, vol, sino, st, f64);
+    par_fp(*par, vol, sino, st, f64, acc_in, norm);
 }
 
 size_t Projector::bp_scratch_floats() const {
   return cone ? cone_dpix_floats(*cone) : par_bp_scratch_floats(*par);
 }
 
-void Projector::bp_scaled(const float *dpix, float *vol, hipStream_t st) const {
-  CCPI_REQUIRE(cone, CCPI_E_ARG, "pre-scaled backprojection is a cone-beam form");
-  cone_bp_scaled(*cone, dpix, vol, st);
+size_t Projector::fp_norm_parts() const {
+  return cone ? cone_fp_norm_parts(*cone) : par_fp_norm_parts(*par);
 }
 
-void Projector::bp(const float *sino, float *vol, hipStream_t st, float *scratch,
-                   bool f64) const {
+size_t Projector::bp_norm_parts() const {
+  return cone ? cone_bp_norm_parts(*cone) : par_bp_norm_parts(*par);
+}
+
+void Projector::bp_scaled(const float *dpix, float *vol, hipStream_t st, bool acc_in,
+                          double *norm) const {
+  CCPI_REQUIRE(cone, CCPI_E_ARG, "pre-scaled backprojection is a cone-beam form");
+  cone_bp_scaled(*cone, dpix, vol, st, false, acc_in, norm);
+}
+
+void Projector::bp(const float *sino, float *vol, hipStream_t st, float *scratch, bool f64,
+                   bool acc_in, double *norm) const {
   auto run = [&](float *scr) {
     if (cone) {
       cone_scale(*cone, sino, scr, st);
-      cone_bp_scaled(*cone, scr, vol, st, f64);
+      cone_bp_scaled(*cone, scr, vol, st, f64, acc_in, norm);
     } else {
-      par_bp(*par, sino, vol, scr, st, f64);
+      par_bp(*par, sino, vol, scr, st, f64, acc_in, norm);
     }
   };
   const size_t need = bp_scratch_floats();

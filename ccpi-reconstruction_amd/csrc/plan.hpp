@@ -170,17 +170,22 @@ struct BpLayout {
 };
 BpLayout cone_bp_layout(const Geometry &g);
 
-// launchers (cone.hip, parallel.hip)
+// launchers (cone.hip, parallel.hip).  f64: accumulate the projector sums
+// in double (the FP64-accumulation option of the solvers); the default is
+// the reference's float sums.  acc_in: add to the output (the reference's
+// f2D / b2D) or overwrite it.  norm: null, or *_norm_parts(P) doubles that
+// receive FP64 partial sums of the squares of the output (its ||.||^2 is
+// their sum, in index order; the epilogue norms of common.hpp).
 void cone_build_plan(ConePlan &P, hipStream_t st);
-// f64: accumulate the projector sums in double (the FP64-accumulation
-// option of the solvers); the default is the reference's float sums
 void cone_fp(const ConePlan &P, const float *vol, float *sino, hipStream_t st,
-             bool f64 = false);
+             bool f64 = false, bool acc_in = true, double *norm = nullptr);
+size_t cone_fp_norm_parts(const ConePlan &P);
 // dpix holds na * nh rows of args.nvp floats (cone_dpix_floats)
 size_t cone_dpix_floats(const ConePlan &P);
 void cone_scale(const ConePlan &P, const float *pix, float *dpix, hipStream_t st);
 void cone_bp_scaled(const ConePlan &P, const float *dpix, float *vol, hipStream_t st,
-                    bool f64 = false);
+                    bool f64 = false, bool acc_in = true, double *norm = nullptr);
+size_t cone_bp_norm_parts(const ConePlan &P);
 
 void par_build_plan(ParPlan &P, hipStream_t st);
 // Parallel FP without any device allocation: the geometry's tables go into
@@ -188,12 +193,15 @@ void par_build_plan(ParPlan &P, hipStream_t st);
 constexpr size_t kNoAllocScratchBytes = size_t(1) << 20;
 size_t par_noalloc_bytes(const Geometry &g);
 void par_fp_noalloc(const Geometry &g, const float *vol, float *sino, hipStream_t st);
-void par_fp(const ParPlan &P, const float *vol, float *sino, hipStream_t st, bool f64 = false);
+void par_fp(const ParPlan &P, const float *vol, float *sino, hipStream_t st, bool f64 = false,
+            bool acc_in = true, double *norm = nullptr);
+size_t par_fp_norm_parts(const ParPlan &P);
 // scratch: par_bp_scratch_floats(P) floats (partials of the angle groups),
 // unused when that is 0
 void par_bp(const ParPlan &P, const float *sino, float *vol, float *scratch, hipStream_t st,
-            bool f64 = false);
+            bool f64 = false, bool acc_in = true, double *norm = nullptr);
 size_t par_bp_scratch_floats(const ParPlan &P);
+size_t par_bp_norm_parts(const ParPlan &P);
 // nonzero system-matrix entries (gathered voxel operands of one FP)
 void cone_nnz(const ConePlan &P, unsigned long long *count, hipStream_t st);
 void par_nnz(const ParPlan &P, unsigned long long *count, hipStream_t st);
@@ -205,17 +213,23 @@ struct Projector {
   std::shared_ptr<ParPlan> par;
   explicit Projector(const Geometry &g, hipStream_t st);
   // f64: projector sums accumulated in double (the solvers' FP64-
-  // accumulation option); default: the reference's float sums
-  void fp(const float *vol, float *sino, hipStream_t st, bool f64 = false) const;
+  // accumulation option); default: the reference's float sums.  acc_in /
+  // norm: as the launchers above (norm: fp_norm_parts() / bp_norm_parts()
+  // doubles)
+  void fp(const float *vol, float *sino, hipStream_t st, bool f64 = false, bool acc_in = true,
+          double *norm = nullptr) const;
   // scratch: bp_scratch_floats() floats owned by the caller and used by
   // no other call meanwhile (a solver's workspace); nullptr takes the
   // plan's shared scratch
   void bp(const float *sino, float *vol, hipStream_t st, float *scratch = nullptr,
-          bool f64 = false) const;
+          bool f64 = false, bool acc_in = true, double *norm = nullptr) const;
   // BP of an already scaled sinogram in the padded BP layout (cone only):
   // a solver's fused update kernels write it
-  void bp_scaled(const float *dpix, float *vol, hipStream_t st) const;
+  void bp_scaled(const float *dpix, float *vol, hipStream_t st, bool acc_in = true,
+                 double *norm = nullptr) const;
   size_t bp_scratch_floats() const;
+  size_t fp_norm_parts() const;
+  size_t bp_norm_parts() const;
   const Geometry &geometry() const { return cone ? cone->g : par->g; }
 };
 

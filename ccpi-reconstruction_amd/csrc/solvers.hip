@@ -14,6 +14,7 @@
 //     ratio then clamp; x * N * t then clamp).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "common.hpp"
@@ -60,11 +61,12 @@ __global__ __launch_bounds__(kRedThreads) void sumsq_kernel(const float *__restr
 }
 
 template <class Acc>
-__device__ Acc sum_partials(const Acc *part) {
-  // fixed-order pairwise tree over kRedBlocks partials in one block
+__device__ Acc sum_partials(const Acc *part, int n) {
+  // n partials in one block: each thread a fixed strided set in order, then
+  // a fixed-order pairwise tree
   __shared__ Acc sh[kRedThreads];
   Acc s = 0;
-  for (int q = threadIdx.x; q < kRedBlocks; q += kRedThreads) s += part[q];
+  for (int q = threadIdx.x; q < n; q += kRedThreads) s += part[q];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int w = kRedThreads / 2; w > 0; w >>= 1) {
@@ -76,9 +78,9 @@ __device__ Acc sum_partials(const Acc *part) {
 
 // gamma = ||d||^2 at start (cgls.cpp:48-49)
 template <class Acc>
-__global__ __launch_bounds__(kRedThreads) void cgls_init_kernel(const Acc *part,
+__global__ __launch_bounds__(kRedThreads) void cgls_init_kernel(const Acc *part, int n,
                                                                 CglsScalars *sc) {
-  const Acc s = sum_partials(part);
+  const Acc s = sum_partials(part, n);
   if (threadIdx.x == 0) {
     sc->gamma = (double)s;
     sc->gamma_f = (float)s;
@@ -87,9 +89,9 @@ __global__ __launch_bounds__(kRedThreads) void cgls_init_kernel(const Acc *part,
 
 // alpha = gamma / ||Ad||^2 (cgls.cpp:118-119)
 template <class Acc>
-__global__ __launch_bounds__(kRedThreads) void cgls_alpha_kernel(const Acc *part,
+__global__ __launch_bounds__(kRedThreads) void cgls_alpha_kernel(const Acc *part, int n,
                                                                  CglsScalars *sc, int fp32) {
-  const Acc s = sum_partials(part);
+  const Acc s = sum_partials(part, n);
   if (threadIdx.x == 0) {
     if (fp32) {
       const float a = sc->gamma_f / (float)s;
@@ -104,11 +106,11 @@ __global__ __launch_bounds__(kRedThreads) void cgls_alpha_kernel(const Acc *part
 
 // beta = ||s||^2 / gamma, gamma <- ||s||^2, record ||A^T r|| (cgls.cpp:153-156)
 template <class Acc>
-__global__ __launch_bounds__(kRedThreads) void cgls_beta_kernel(const Acc *part,
+__global__ __launch_bounds__(kRedThreads) void cgls_beta_kernel(const Acc *part, int n,
                                                                 CglsScalars *sc,
                                                                 double *norms, int it,
                                                                 int fp32) {
-  const Acc s = sum_partials(part);
+  const Acc s = sum_partials(part, n);
   if (threadIdx.x == 0) {
     if (fp32) {
       const float nn = (float)s;
@@ -129,12 +131,13 @@ __global__ __launch_bounds__(kRedThreads) void cgls_beta_kernel(const Acc *part,
 // mode the two float norms are added in float as the reference does
 template <class Acc>
 __global__ __launch_bounds__(kRedThreads) void cgls_alpha_reg_kernel(const Acc *part_pix,
+                                                                     int n_pix,
                                                                      const Acc *part_vox,
                                                                      CglsScalars *sc,
                                                                      int fp32) {
-  const Acc sp = sum_partials(part_pix);
+  const Acc sp = sum_partials(part_pix, n_pix);
   __syncthreads();
-  const Acc sv = sum_partials(part_vox);
+  const Acc sv = sum_partials(part_vox, kRedBlocks);
   if (threadIdx.x == 0) {
     if (fp32) {
       const float den = (float)sp + (float)sv;
@@ -300,7 +303,8 @@ void Workspace::ensure(const Projector &P) {
   v3.alloc(nvox * 4);
   p1.alloc(npix * 4);
   p2.alloc(npix * 4);
-  part.alloc(kRedBlocks * sizeof(double));
+  part.alloc(std::max<size_t>({(size_t)kRedBlocks, P.fp_norm_parts(), P.bp_norm_parts()}) *
+             sizeof(double));
   scal.alloc(sizeof(CglsScalars));
 }
 
@@ -316,16 +320,18 @@ void CglsRun::start(hipStream_t st) {
   W.ensure(P);
   norms_dev.alloc((size_t)std::max(iters, 1) * sizeof(double));
   float *d = W.v1.as<float>();
-  dev_fill(d, nvox, 0.f, st);
-  P.bp(b, d, st, W.bp_scratch());
   if (fp32) {
+    // the reference's float partial sums: a separate pass over d
+    dev_fill(d, nvox, 0.f, st);
+    P.bp(b, d, st, W.bp_scratch());
     sumsq<float>(d, nvox, W.part.p, st);
     hipLaunchKernelGGL(cgls_init_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
-                       W.part.as<float>(), W.scal.as<CglsScalars>());
+                       W.part.as<float>(), kRedBlocks, W.scal.as<CglsScalars>());
   } else {
-    sumsq<double>(d, nvox, W.part.p, st);
+    // d = A^T b written over d, its FP64 norm partials from the BP's epilogue
+    P.bp(b, d, st, W.bp_scratch(), false, false, W.part.as<double>());
     hipLaunchKernelGGL(cgls_init_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
-                       W.part.as<double>(), W.scal.as<CglsScalars>());
+                       W.part.as<double>(), (int)P.bp_norm_parts(), W.scal.as<CglsScalars>());
   }
   CCPI_HIP(hipGetLastError());
 }
@@ -357,18 +363,22 @@ void CglsRun::iterate(int it, hipStream_t st) {
   const size_t nvox = P.geometry().n_vox(), npix = P.geometry().n_pix();
   float *d = W.v1.as<float>(), *s = W.v2.as<float>(), *Ad = W.p1.as<float>();
   CglsScalars *sc = W.scal.as<CglsScalars>();
-  dev_fill(Ad, npix, 0.f, st);
+  // FP64 mode: the projectors write their outputs over Ad and s and leave
+  // the FP64 norm partials of ||Ad||^2 and ||s||^2 in W.part (epilogue
+  // norms); FP32 mode keeps the reference's float partial sums, a pass of
+  // their own
+  double *norm = fp32 ? nullptr : W.part.as<double>();
+  if (fp32) dev_fill(Ad, npix, 0.f, st);
   if (log) log->mark(log->fp, st);
-  P.fp(d, Ad, st);
+  P.fp(d, Ad, st, false, fp32, norm);
   if (log) log->mark(log->fp, st);
   if (fp32) {
     sumsq<float>(Ad, npix, W.part.p, st);
     hipLaunchKernelGGL(cgls_alpha_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
-                       W.part.as<float>(), sc, 1);
+                       W.part.as<float>(), kRedBlocks, sc, 1);
   } else {
-    sumsq<double>(Ad, npix, W.part.p, st);
     hipLaunchKernelGGL(cgls_alpha_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
-                       W.part.as<double>(), sc, 0);
+                       W.part.as<double>(), (int)P.fp_norm_parts(), sc, 0);
   }
   if (P.cone) {
     // the residual update writes the cone BP's scaled input as well
@@ -378,26 +388,26 @@ void CglsRun::iterate(int it, hipStream_t st) {
     hipLaunchKernelGGL(cgls_update_xr_scale_kernel,
                        dim3(grid_for(std::max(nvox, nrays * (size_t)A.nvp))), dim3(256), 0, st,
                        x, d, nvox, b, Ad, nrays, A.nh, A.nv, A.nvp, A.dconv, dpix, sc);
-    dev_fill(s, nvox, 0.f, st);
+    if (fp32) dev_fill(s, nvox, 0.f, st);
     if (log) log->mark(log->bp, st);
-    P.bp_scaled(dpix, s, st);
+    P.bp_scaled(dpix, s, st, fp32, norm);
     if (log) log->mark(log->bp, st);
   } else {
     hipLaunchKernelGGL(cgls_update_xr_kernel, dim3(grid_for(std::max(nvox, npix))), dim3(256),
                        0, st, x, d, nvox, b, Ad, npix, sc);
-    dev_fill(s, nvox, 0.f, st);
+    if (fp32) dev_fill(s, nvox, 0.f, st);
     if (log) log->mark(log->bp, st);
-    P.bp(b, s, st, W.bp_scratch());
+    P.bp(b, s, st, W.bp_scratch(), false, fp32, norm);
     if (log) log->mark(log->bp, st);
   }
   if (fp32) {
     sumsq<float>(s, nvox, W.part.p, st);
     hipLaunchKernelGGL(cgls_beta_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
-                       W.part.as<float>(), sc, norms_dev.as<double>(), it, 1);
+                       W.part.as<float>(), kRedBlocks, sc, norms_dev.as<double>(), it, 1);
   } else {
-    sumsq<double>(s, nvox, W.part.p, st);
     hipLaunchKernelGGL(cgls_beta_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
-                       W.part.as<double>(), sc, norms_dev.as<double>(), it, 0);
+                       W.part.as<double>(), (int)P.bp_norm_parts(), sc, norms_dev.as<double>(),
+                       it, 0);
   }
   hipLaunchKernelGGL(cgls_update_d_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, d, s, nvox,
                      sc);
@@ -444,49 +454,53 @@ void run_cgls_reg(const Projector &P, float *b, float *x, int iters, int kind, d
   CglsScalars *sc = W.scal.as<CglsScalars>();
   const float lam = (float)lambda;
   const int gv = grid_for(nvox);
-  dev_fill(d, nvox, 0.f, st);
+  // FP64 mode: projector outputs written over d, Ad, s with their norm
+  // partials from the projectors' epilogues, as in run_cgls (so lambda = 0
+  // gives run_cgls bit for bit); FP32 mode: float partial sums in passes of
+  // their own
+  double *norm = fp32 ? nullptr : W.part.as<double>();
   dev_fill(adv, nvox, 0.f, st);
-  P.bp(b, d, st, W.bp_scratch());
+  if (fp32) dev_fill(d, nvox, 0.f, st);
+  P.bp(b, d, st, W.bp_scratch(), false, fp32, norm);
   if (fp32) {
     sumsq<float>(d, nvox, W.part.p, st);
     hipLaunchKernelGGL(cgls_init_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
-                       W.part.as<float>(), sc);
+                       W.part.as<float>(), kRedBlocks, sc);
   } else {
-    sumsq<double>(d, nvox, W.part.p, st);
     hipLaunchKernelGGL(cgls_init_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
-                       W.part.as<double>(), sc);
+                       W.part.as<double>(), (int)P.bp_norm_parts(), sc);
   }
   for (int it = 0; it < iters; it++) {
-    dev_fill(Ad, npix, 0.f, st);
-    P.fp(d, Ad, st);
+    if (fp32) dev_fill(Ad, npix, 0.f, st);
+    P.fp(d, Ad, st, false, fp32, norm);
     reg_term(kind, d, L, scr, G.nx, G.ny, G.nz, st);
     if (fp32) {
       sumsq<float>(Ad, npix, W.part.p, st);
       hipLaunchKernelGGL(axpy_sumsq_kernel<float>, dim3(kRedBlocks), dim3(kRedThreads), 0, st,
                          adv, L, nvox, lam, W.part2.as<float>());
       hipLaunchKernelGGL(cgls_alpha_reg_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
-                         W.part.as<float>(), W.part2.as<float>(), sc, 1);
+                         W.part.as<float>(), kRedBlocks, W.part2.as<float>(), sc, 1);
     } else {
-      sumsq<double>(Ad, npix, W.part.p, st);
       hipLaunchKernelGGL(axpy_sumsq_kernel<double>, dim3(kRedBlocks), dim3(kRedThreads), 0,
                          st, adv, L, nvox, lam, W.part2.as<double>());
       hipLaunchKernelGGL(cgls_alpha_reg_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
-                         W.part.as<double>(), W.part2.as<double>(), sc, 0);
+                         W.part.as<double>(), (int)P.fp_norm_parts(), W.part2.as<double>(),
+                         sc, 0);
     }
     hipLaunchKernelGGL(cgls_update_xr_kernel, dim3(grid_for(std::max(nvox, npix))), dim3(256),
                        0, st, x, d, nvox, b, Ad, npix, sc);
     reg_term(kind, x, L, scr, G.nx, G.ny, G.nz, st);
     hipLaunchKernelGGL(axpy_kernel, dim3(gv), dim3(256), 0, st, x, L, nvox, lam);
-    dev_fill(s, nvox, 0.f, st);
-    P.bp(b, s, st, W.bp_scratch());
+    if (fp32) dev_fill(s, nvox, 0.f, st);
+    P.bp(b, s, st, W.bp_scratch(), false, fp32, norm);
     if (fp32) {
       sumsq<float>(s, nvox, W.part.p, st);
       hipLaunchKernelGGL(cgls_beta_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
-                         W.part.as<float>(), sc, norms_dev.as<double>(), it, 1);
+                         W.part.as<float>(), kRedBlocks, sc, norms_dev.as<double>(), it, 1);
     } else {
-      sumsq<double>(s, nvox, W.part.p, st);
       hipLaunchKernelGGL(cgls_beta_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
-                         W.part.as<double>(), sc, norms_dev.as<double>(), it, 0);
+                         W.part.as<double>(), (int)P.bp_norm_parts(), sc,
+                         norms_dev.as<double>(), it, 0);
     }
     hipLaunchKernelGGL(cgls_update_d_kernel, dim3(gv), dim3(256), 0, st, d, s, nvox, sc);
     CCPI_HIP(hipGetLastError());
