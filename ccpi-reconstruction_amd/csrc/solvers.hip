@@ -60,6 +60,20 @@ __global__ __launch_bounds__(kRedThreads) void sumsq_kernel(const float *__restr
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// Folds n FP64 partials into kRedBlocks: workgroup b sums the contiguous
+// range b of them (its threads strided over it, then block_sum), so the
+// result is a fixed function of the partials.
+__global__ __launch_bounds__(kRedThreads) void fold_partials_kernel(const double *__restrict__ part,
+                                                                    size_t n,
+                                                                    double *__restrict__ out) {
+  const size_t chunk = (n + kRedBlocks - 1) / kRedBlocks;
+  const size_t q0 = (size_t)blockIdx.x * chunk, q1 = q0 + chunk < n ? q0 + chunk : n;
+  double s = 0.0;
+  for (size_t q = q0 + threadIdx.x; q < q1; q += kRedThreads) s += part[q];
+  s = block_sum(s);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
 template <class Acc>
 __device__ Acc sum_partials(const Acc *part, int n) {
   // n partials in one block: each thread a fixed strided set in order, then
@@ -305,7 +319,22 @@ void Workspace::ensure(const Projector &P) {
   p2.alloc(npix * 4);
   part.alloc(std::max<size_t>({(size_t)kRedBlocks, P.fp_norm_parts(), P.bp_norm_parts()}) *
              sizeof(double));
+  fold.alloc(kRedBlocks * sizeof(double));
   scal.alloc(sizeof(CglsScalars));
+}
+
+// The projector epilogues leave one norm partial per workgroup (or column):
+// more than one reduction workgroup sums quickly past kFoldAbove, where
+// they are first folded into kRedBlocks (into `folded`).  Returns the
+// partials to reduce and sets n to their count.
+constexpr size_t kFoldAbove = 16 * kRedThreads;
+static const double *fold_parts(const double *part, size_t &n, double *folded, hipStream_t st) {
+  if (n <= kFoldAbove) return part;
+  hipLaunchKernelGGL(fold_partials_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, st, part, n,
+                     folded);
+  CCPI_HIP(hipGetLastError());
+  n = kRedBlocks;
+  return folded;
 }
 
 template <class Acc>
@@ -330,8 +359,10 @@ void CglsRun::start(hipStream_t st) {
   } else {
     // d = A^T b written over d, its FP64 norm partials from the BP's epilogue
     P.bp(b, d, st, W.bp_scratch(), false, false, W.part.as<double>());
-    hipLaunchKernelGGL(cgls_init_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
-                       W.part.as<double>(), (int)P.bp_norm_parts(), W.scal.as<CglsScalars>());
+    size_t n = P.bp_norm_parts();
+    const double *pp = fold_parts(W.part.as<double>(), n, W.fold.as<double>(), st);
+    hipLaunchKernelGGL(cgls_init_kernel<double>, dim3(1), dim3(kRedThreads), 0, st, pp, (int)n,
+                       W.scal.as<CglsScalars>());
   }
   CCPI_HIP(hipGetLastError());
 }
@@ -377,8 +408,10 @@ void CglsRun::iterate(int it, hipStream_t st) {
     hipLaunchKernelGGL(cgls_alpha_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
                        W.part.as<float>(), kRedBlocks, sc, 1);
   } else {
-    hipLaunchKernelGGL(cgls_alpha_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
-                       W.part.as<double>(), (int)P.fp_norm_parts(), sc, 0);
+    size_t n = P.fp_norm_parts();
+    const double *pp = fold_parts(W.part.as<double>(), n, W.fold.as<double>(), st);
+    hipLaunchKernelGGL(cgls_alpha_kernel<double>, dim3(1), dim3(kRedThreads), 0, st, pp, (int)n,
+                       sc, 0);
   }
   if (P.cone) {
     // the residual update writes the cone BP's scaled input as well
@@ -405,9 +438,10 @@ void CglsRun::iterate(int it, hipStream_t st) {
     hipLaunchKernelGGL(cgls_beta_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
                        W.part.as<float>(), kRedBlocks, sc, norms_dev.as<double>(), it, 1);
   } else {
-    hipLaunchKernelGGL(cgls_beta_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
-                       W.part.as<double>(), (int)P.bp_norm_parts(), sc, norms_dev.as<double>(),
-                       it, 0);
+    size_t n = P.bp_norm_parts();
+    const double *pp = fold_parts(W.part.as<double>(), n, W.fold.as<double>(), st);
+    hipLaunchKernelGGL(cgls_beta_kernel<double>, dim3(1), dim3(kRedThreads), 0, st, pp, (int)n,
+                       sc, norms_dev.as<double>(), it, 0);
   }
   hipLaunchKernelGGL(cgls_update_d_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, d, s, nvox,
                      sc);
@@ -467,8 +501,10 @@ void run_cgls_reg(const Projector &P, float *b, float *x, int iters, int kind, d
     hipLaunchKernelGGL(cgls_init_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
                        W.part.as<float>(), kRedBlocks, sc);
   } else {
-    hipLaunchKernelGGL(cgls_init_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
-                       W.part.as<double>(), (int)P.bp_norm_parts(), sc);
+    size_t n = P.bp_norm_parts();
+    const double *pp = fold_parts(W.part.as<double>(), n, W.fold.as<double>(), st);
+    hipLaunchKernelGGL(cgls_init_kernel<double>, dim3(1), dim3(kRedThreads), 0, st, pp, (int)n,
+                       sc);
   }
   for (int it = 0; it < iters; it++) {
     if (fp32) dev_fill(Ad, npix, 0.f, st);
@@ -483,9 +519,10 @@ void run_cgls_reg(const Projector &P, float *b, float *x, int iters, int kind, d
     } else {
       hipLaunchKernelGGL(axpy_sumsq_kernel<double>, dim3(kRedBlocks), dim3(kRedThreads), 0,
                          st, adv, L, nvox, lam, W.part2.as<double>());
-      hipLaunchKernelGGL(cgls_alpha_reg_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
-                         W.part.as<double>(), (int)P.fp_norm_parts(), W.part2.as<double>(),
-                         sc, 0);
+      size_t n = P.fp_norm_parts();
+      const double *pp = fold_parts(W.part.as<double>(), n, W.fold.as<double>(), st);
+      hipLaunchKernelGGL(cgls_alpha_reg_kernel<double>, dim3(1), dim3(kRedThreads), 0, st, pp,
+                         (int)n, W.part2.as<double>(), sc, 0);
     }
     hipLaunchKernelGGL(cgls_update_xr_kernel, dim3(grid_for(std::max(nvox, npix))), dim3(256),
                        0, st, x, d, nvox, b, Ad, npix, sc);
@@ -498,9 +535,10 @@ void run_cgls_reg(const Projector &P, float *b, float *x, int iters, int kind, d
       hipLaunchKernelGGL(cgls_beta_kernel<float>, dim3(1), dim3(kRedThreads), 0, st,
                          W.part.as<float>(), kRedBlocks, sc, norms_dev.as<double>(), it, 1);
     } else {
-      hipLaunchKernelGGL(cgls_beta_kernel<double>, dim3(1), dim3(kRedThreads), 0, st,
-                         W.part.as<double>(), (int)P.bp_norm_parts(), sc,
-                         norms_dev.as<double>(), it, 0);
+      size_t n = P.bp_norm_parts();
+      const double *pp = fold_parts(W.part.as<double>(), n, W.fold.as<double>(), st);
+      hipLaunchKernelGGL(cgls_beta_kernel<double>, dim3(1), dim3(kRedThreads), 0, st, pp,
+                         (int)n, sc, norms_dev.as<double>(), it, 0);
     }
     hipLaunchKernelGGL(cgls_update_d_kernel, dim3(gv), dim3(256), 0, st, d, s, nvox, sc);
     CCPI_HIP(hipGetLastError());

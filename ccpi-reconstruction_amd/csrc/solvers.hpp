@@ -20,7 +20,8 @@ struct CglsScalars {
 struct Workspace {
   DeviceBuffer v1, v2, v3;   // volume-sized
   DeviceBuffer p1, p2;       // sinogram-sized
-  DeviceBuffer part;         // reduction partials
+  DeviceBuffer part;         // reduction partials (and projector norm partials)
+  DeviceBuffer fold;         // norm partials folded to kRedBlocks
   DeviceBuffer scal;         // CglsScalars
   DeviceBuffer v4, part2;    // regularised CGLS: Ad_vox, its partials
   DeviceBuffer tvd;          // TV term: D1, D2, D3
