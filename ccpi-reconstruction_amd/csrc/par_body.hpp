@@ -252,8 +252,9 @@ __device__ inline void par_cells(const ParArgs &g, int r, int a, int h, Emit &&e
   }
 }
 
-// Cells walked per group in the one-row-per-slice FP: the group's column
-// reads are all in flight before the in-order accumulation
+// Cells per group of the FP's walk (or walk-table reads): in the
+// one-row-per-slice FP the group's column reads are all in flight before
+// the in-order accumulation
 constexpr int kParFpGroup = 8;
 
 // FP of detector rows [v0, v0 + RC) of ray r (clipped to nv): `acc_in`

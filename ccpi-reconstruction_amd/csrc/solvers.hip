@@ -323,9 +323,9 @@ void Workspace::ensure(const Projector &P) {
   scal.alloc(sizeof(CglsScalars));
 }
 
-// The projector epilogues leave one norm partial per workgroup (or column):
-// more than one reduction workgroup sums quickly past kFoldAbove, where
-// they are first folded into kRedBlocks (into `folded`).  Returns the
+// The projector epilogues leave one norm partial per workgroup (or column).
+// The single-workgroup reductions sum up to kFoldAbove of them quickly;
+// more are first folded into kRedBlocks (into `folded`).  Returns the
 // partials to reduce and sets n to their count.
 constexpr size_t kFoldAbove = 16 * kRedThreads;
 static const double *fold_parts(const double *part, size_t &n, double *folded, hipStream_t st) {
