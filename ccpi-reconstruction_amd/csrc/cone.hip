@@ -260,21 +260,23 @@ __device__ inline int med3_0s(int x, int hi) {
   return r;
 }
 
+// v_med3_i32 with lo = -2 (inline constant) and a uniform hi
+__device__ inline int med3_m2s(int x, int hi) {
+  int r;
+  asm("v_med3_i32 %0, %1, -2, %2" : "=v"(r) : "v"(x), "s"(hi));
+  return r;
+}
+
 // LDS byte address of a shared-memory pointer, and back
 typedef __attribute__((address_space(3))) float lds_f32;
 __device__ inline uint32_t lds_addr(const float *p) {
   return (uint32_t)(uintptr_t)(const lds_f32 *)p;
 }
 __device__ inline lds_f32 *lds_ptr(uint32_t a) { return (lds_f32 *)(uintptr_t)a; }
-// (k << 2) + base in one v_lshl_add_u32 (per-lane / uniform base)
+// (k << 2) + base in one v_lshl_add_u32
 __device__ inline uint32_t k4_add(int k, uint32_t base) {
   uint32_t r;
   asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(r) : "v"(k), "v"(base));
-  return r;
-}
-__device__ inline uint32_t k4_add_s(int k, uint32_t base) {
-  uint32_t r;
-  asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(r) : "v"(k), "s"(base));
   return r;
 }
 
@@ -480,6 +482,11 @@ constexpr int kBpTile = 8;
 // 4 -> 93.9 ms, 6 -> 92.7, 8 -> 93.0 (spills), 12 -> 119.9 (spills))
 template <int S>
 constexpr int bp_recs() { return S <= 2 ? 6 : (S <= 4 ? 4 : (S <= 8 ? 2 : 1)); }
+// Trash cells on each side of a BP column array (cells -kBpPad .. -1 and
+// nz .. nz + kBpPad - 1 take the contributions of slices outside the
+// volume); the slice-plane table has the same margin, clamped to planes 0
+// and nz.
+constexpr int kBpPad = 4;
 
 // Detector columns covered by the four corner lines of one voxel column
 // (c2D.cpp:837-901).
@@ -712,7 +719,7 @@ __device__ inline BpSlot bp_slot(const ConeArgs &g, int v, int pass, bool one_se
 // fall on distinct slices, and so do their second slices; slot t has its own
 // LDS array, so the slots' first-slice updates are plain read-add-writes
 // over distinct cells, issued together, followed by their second-slice
-// updates.  Cells 0 and nz + 1
+// updates.  The kBpPad cells on each side
 // of an array are trash for slices outside the volume.  PHASED (rows per
 // slice beyond 4, or arrays beyond the LDS budget): S = 4, 8, 12 or 16 slots
 // and one array; the slots update one after another, each a read-add-write
@@ -748,7 +755,7 @@ __device__ inline int bp_col_phases(const ConeArgs &g, const BpCol &C) {
 
 // One block of the phased form: lane L holds rows vb + S*L + t, t < S;
 // every record's S rows are computed and written slot by slot into the one
-// array (cells acc1[-1] and acc1[nz] are trash).  MAIN: every row of the
+// array (cells acc1[-1] and acc1[nz] take the trash).  MAIN: every row of the
 // block below nv is a main row (as in the slot form's main path); otherwise
 // each row takes its row set's form, pass by pass when the sets overlap (as
 // bp_slot).
@@ -879,12 +886,14 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
   constexpr bool kFloatAcc = std::is_same<AccT, float>::value;
   extern __shared__ float smem[];
   const int nz = g.nz, nv = g.nv;
-  // slice planes with plane 0 repeated in front: s_vzx[q + 1] = plane max(q, 0)
+  // slice planes, clamped at both ends: s_vz1[q] = plane min(max(q, 0), nz)
+  // for q in [-kBpPad, nz + kBpPad)
   float *s_vzx = smem;
-  for (int q = threadIdx.x; q <= nz + 1; q += 64 * kBpWaves) s_vzx[q] = g.vz[max(q - 1, 0)];
+  for (int q = threadIdx.x; q < nz + 2 * kBpPad; q += 64 * kBpWaves)
+    s_vzx[q] = g.vz[min(max(q - kBpPad, 0), nz)];
   __syncthreads();
-  const float *s_vz1 = s_vzx + 1;
-  const int astride = lds_pad4(nz + 2);
+  const float *s_vz1 = s_vzx + kBpPad;
+  const int astride = lds_pad4(nz + 2 * kBpPad);
   // the wave index is uniform: keep the column's loads scalar
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
@@ -915,11 +924,11 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
     }
     return;
   }
-  AccT *acc = reinterpret_cast<AccT *>(smem + lds_pad4(nz + 2)) + wave * narr * astride;
+  AccT *acc = reinterpret_cast<AccT *>(smem + astride) + wave * narr * astride;
   for (int q = lane; q < narr * astride; q += 64) acc[q] = (AccT)0;
   __builtin_amdgcn_wave_barrier();
   if (acc_in)
-    for (int k = lane; k < nz; k += 64) acc[k + 1] = vcol[k];
+    for (int k = lane; k < nz; k += 64) acc[k + kBpPad] = vcol[k];
   __builtin_amdgcn_wave_barrier();
 
   const int min_v = C.min_v, max_v = C.max_v;
@@ -938,29 +947,39 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
   const int nvp4 = 4 * g.nvp;
   for (int vb = 0; vb < nv; vb += span) {
     if constexpr (PHASED) {
-      bp_phased_block<S, AccT>(g, R, C.n, dpix, s_vz1, acc + 1, vb, lane, nph, ph, inv_dz,
+      bp_phased_block<S, AccT>(g, R, C.n, dpix, s_vz1, acc + kBpPad, vb, lane, nph, ph, inv_dz,
                          one_set && g.bp_main_safe && vb >= min_v && min(vb + span, nv) <= max_v,
                          one_set, min_v, max_v);
       continue;
-    } else if (kFloatAcc && one_set && g.bp_main_safe && vb >= min_v && vb + span <= max_v) {
-      // Every row of the block is a main row: its ray stays inside the
-      // volume's z range for all of this column's rays (c2D.cpp:657-676), so
-      // k is in [0, nz) (the clamp only guards float ties) and k +- 1 in
-      // [-1, nz], where cells -1 and nz are trash.  No row-set tests.
-      float dzv[S], izv[S];
-      uint32_t b1[S], bz[S];
-      uint32_t b0[S];  // uniform
+    } else if (kFloatAcc && one_set && g.bp_main_safe) {
+      // Every row in one row set (c2D.cpp:657-749): the floor form below
+      // min_v (set 0), the truncating form from there on, the second slice
+      // below the row's first under the source row g.mid and above it from
+      // there on.  A lane's slot constants fold its row's set into
+      // addresses: with k = (int)(kb + ai * dz) clamped to [-2, nz + 2],
+      // the first slice k - ks is at b0 + 4k, the second at b1 + 4k and the
+      // boundary plane (k - ks under the source row, k + 1 from it) at
+      // bp + 4k.  Those cells stay within the kBpPad trash cells on each
+      // side and the plane table's clamped ends, and the real cells they
+      // reach are those of the clamped row-set form below, with the same
+      // weights: a k past either clamp sends both updates to trash there
+      // too.  Block rows past nv write trash (kb far below slice 0).
+      float kb[S], dzv[S], izv[S];
+      uint32_t b0[S], b1[S], bz[S];
 #pragma unroll
       for (int t = 0; t < S; ++t) {
         const int v = vb + S * lane + t;
-        const bool lower = v < g.mid;
-        dzv[t] = g.dz[v];
-        izv[t] = g.iz[v];
-        b0[t] = __builtin_amdgcn_readfirstlane(lds_addr(reinterpret_cast<const float *>(acc + t * astride + 1)));
-        b1[t] = b0[t] + (lower ? -4 : 4);
-        bz[t] = lds_addr(s_vz1 + (lower ? 0 : 1));
+        const int vv = min(v, nv - 1);
+        const bool lower = v < g.mid, floor_form = v < min_v;
+        kb[t] = v < nv ? (floor_form ? g.pzbz1 : g.pzbz) : kBpOff;
+        dzv[t] = g.dz[vv];
+        izv[t] = g.iz[vv];
+        const uint32_t ks4 = floor_form ? 4u : 0u;
+        b0[t] = lds_addr(reinterpret_cast<const float *>(acc + t * astride + kBpPad)) - ks4;
+        b1[t] = lower ? b0[t] - 4u : b0[t] + 4u;
+        bz[t] = lds_addr(s_vz1) - ks4 + (lower ? 0u : 4u);
       }
-      const int nzm1 = nz - 1;
+      const int khi = nz + 2;
       auto live = [&](const BpRec &rec) {
         return vb < (int)(rec.w >> 16) && vb + span > (int)(rec.w & 0xffffu);
       };
@@ -972,20 +991,35 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
           p[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
                                               prow, (int)lane_off, 4 * (vb + t), 0));
       };
-      auto update = [&](const BpRec &rec, const float *p) {
+      // DIR -1 / +1: every row of the block under / from the source row,
+      // the second slice one cell below / above the first (an immediate
+      // offset); 0: both kinds of rows
+      auto update = [&](const BpRec &rec, const float *p, auto DIR) {
+        constexpr int dir = decltype(DIR)::value;
         const float a0 = __builtin_bit_cast(float, rec.y);
         const float a1 = __builtin_bit_cast(float, rec.z);
         const float ai = a0 * inv_dz;
         // the S slots write S separate arrays: their first-slice updates
         // go together, then (after the lanes' first slices are written)
-        // their second-slice updates
+        // their second-slice updates.  A row whose segment stays in one
+        // slice (mz = a1) has c1 = 0 and leaves its second slice alone:
+        // adding a zero changes no cell, and the lanes left out of an LDS
+        // round take no bank cycles (most segments cross no plane)
         lds_f32 *c0p[S], *c1p[S];
-        float c0[S], c1[S], o0[S], o1[S];
+        float c0[S], c1[S], o0[S];
 #pragma unroll
         for (int t = 0; t < S; ++t) {
-          const int k = med3_0s((int)(g.pzbz + ai * dzv[t]), nzm1);
-          c0p[t] = lds_ptr(k4_add_s(k, b0[t]));
-          c1p[t] = lds_ptr(k4_add(k, b1[t]));
+          const int k = med3_m2s((int)(kb[t] + ai * dzv[t]), khi);
+          if constexpr (dir < 0) {
+            c1p[t] = lds_ptr(k4_add(k, b1[t]));
+            c0p[t] = c1p[t] + 1;
+          } else if constexpr (dir > 0) {
+            c0p[t] = lds_ptr(k4_add(k, b0[t]));
+            c1p[t] = c0p[t] + 1;
+          } else {
+            c0p[t] = lds_ptr(k4_add(k, b0[t]));
+            c1p[t] = lds_ptr(k4_add(k, b1[t]));
+          }
           const float az = *lds_ptr(k4_add(k, bz[t])) * izv[t];
           const float mz = min_sv(a1, az);
           c0[t] = p[t] * (mz - a0);
@@ -997,35 +1031,46 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
         for (int t = 0; t < S; ++t) *c0p[t] = o0[t] + c0[t];
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int t = 0; t < S; ++t) o1[t] = *c1p[t];
-#pragma unroll
-        for (int t = 0; t < S; ++t) *c1p[t] = o1[t] + c1[t];
+        for (int t = 0; t < S; ++t)
+          if (c1[t] != 0.0f) *c1p[t] = *c1p[t] + c1[t];
         __builtin_amdgcn_wave_barrier();
       };
       // kBpRecs records at a time: all their gathers are in flight before
-      // the first update (the updates stay in record order)
-      int e = 0;
-      constexpr int kBpRecs = bp_recs<S>();
-      for (; e + kBpRecs <= C.n; e += kBpRecs) {
-        BpRec r[kBpRecs];
-        float p[kBpRecs][S];
+      // the first update (the updates stay in record order); the live
+      // tests go into one uniform bit mask
+      auto sweep = [&](auto DIR) {
+        int e = 0;
+        constexpr int kBpRecs = bp_recs<S>();
+        for (; e + kBpRecs <= C.n; e += kBpRecs) {
+          BpRec r[kBpRecs];
+          float p[kBpRecs][S];
+          uint32_t lm = 0;
 #pragma unroll
-        for (int u = 0; u < kBpRecs; ++u) r[u] = R[e + u];
+          for (int u = 0; u < kBpRecs; ++u) r[u] = R[e + u];
 #pragma unroll
-        for (int u = 0; u < kBpRecs; ++u)
-          if (live(r[u])) gather(r[u], p[u]);
+          for (int u = 0; u < kBpRecs; ++u) lm |= live(r[u]) ? 1u << u : 0u;
 #pragma unroll
-        for (int u = 0; u < kBpRecs; ++u)
-          if (live(r[u])) update(r[u], p[u]);
-      }
-      for (; e < C.n; ++e) {
-        const BpRec r0 = R[e];
-        float p0[S];
-        if (live(r0)) {
-          gather(r0, p0);
-          update(r0, p0);
+          for (int u = 0; u < kBpRecs; ++u)
+            if (lm & (1u << u)) gather(r[u], p[u]);
+#pragma unroll
+          for (int u = 0; u < kBpRecs; ++u)
+            if (lm & (1u << u)) update(r[u], p[u], DIR);
         }
-      }
+        for (; e < C.n; ++e) {
+          const BpRec r0 = R[e];
+          float p0[S];
+          if (live(r0)) {
+            gather(r0, p0);
+            update(r0, p0, DIR);
+          }
+        }
+      };
+      if (vb + span <= g.mid)
+        sweep(std::integral_constant<int, -1>{});
+      else if (vb >= g.mid)
+        sweep(std::integral_constant<int, 1>{});
+      else
+        sweep(std::integral_constant<int, 0>{});
       continue;
     }
     for (int pass = 0; pass < npass; ++pass) {
@@ -1061,7 +1106,7 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
           const float mz = min_sv(a1, az);
           c0[t] = p[t] * (mz - a0);
           c1[t] = p[t] * (a1 - mz);
-          AccT *ac = acc + t * astride + 1;
+          AccT *ac = acc + t * astride + kBpPad;
           c0p[t] = ac + i0;
           c1p[t] = ac + i1;
         }
@@ -1111,9 +1156,9 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   double ss = 0.0;
   for (int k = lane; k < nz; k += 64) {
-    AccT s = acc[k + 1];
+    AccT s = acc[k + kBpPad];
 #pragma unroll
-    for (int t = 1; t < narr; ++t) s += acc[t * astride + k + 1];
+    for (int t = 1; t < narr; ++t) s += acc[t * astride + k + kBpPad];
     const float o = (float)s;
     vcol[k] = o;
     ss += (double)o * (double)o;
@@ -1151,7 +1196,7 @@ constexpr int kBpMaxStride = 4;
 // LDS of one BP workgroup: the slice planes and `narr` column arrays per
 // wavefront of `acc_bytes`-byte cells.
 static size_t bp_lds_bytes(int nz, int narr, size_t acc_bytes = sizeof(float)) {
-  const size_t pad = (size_t)((nz + 2 + 3) & ~3);
+  const size_t pad = (size_t)((nz + 2 * kBpPad + 3) & ~3);
   return sizeof(float) * pad + acc_bytes * (size_t)kBpWaves * (size_t)narr * pad;
 }
 
