@@ -832,10 +832,11 @@ __device__ void bp_phased_block(const ConeArgs &g, const BpRec *__restrict__ R, 
       const float mz = min_sv(a1, az);
       const float c0 = p[t] * (mz - a0);
       const float c1 = p[t] * (a1 - mz);
+      // a segment inside one slice (c1 = 0) leaves the second alone
       for (int q = 0; q < nph; ++q) {
         if (ph == q) acc1[i0] += c0;
         __builtin_amdgcn_wave_barrier();
-        if (ph == q) acc1[i1] += c1;
+        if (ph == q && c1 != 0.0f) acc1[i1] += c1;
         __builtin_amdgcn_wave_barrier();
       }
     }
@@ -1112,7 +1113,7 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
         }
         {
           // separate arrays per slot: first-slice updates together, then
-          // the second-slice updates
+          // the second-slice updates of the segments that cross a plane
           AccT o[S];
 #pragma unroll
           for (int t = 0; t < S; ++t) o[t] = *c0p[t];
@@ -1120,9 +1121,8 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
           for (int t = 0; t < S; ++t) *c0p[t] = o[t] + c0[t];
           __builtin_amdgcn_wave_barrier();
 #pragma unroll
-          for (int t = 0; t < S; ++t) o[t] = *c1p[t];
-#pragma unroll
-          for (int t = 0; t < S; ++t) *c1p[t] = o[t] + c1[t];
+          for (int t = 0; t < S; ++t)
+            if (c1[t] != 0.0f) *c1p[t] = *c1p[t] + c1[t];
           __builtin_amdgcn_wave_barrier();
         }
       };
