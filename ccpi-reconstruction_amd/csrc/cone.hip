@@ -478,10 +478,11 @@ constexpr int kBpWaves = 4;
 // (BP at 512^3, tiles of 32 / 16 / 8: 92.0 / 90.5 / 90.6 ms; wide cone
 // 768^3: 684 / 674 / 663 ms; 64: 97.6 / 691 ms)
 constexpr int kBpTile = 8;
-// records per step of the BP sweep (512^3 cone, S = 2, at most 64 VGPRs:
-// 4 -> 93.9 ms, 6 -> 92.7, 8 -> 93.0 (spills), 12 -> 119.9 (spills))
+// records per step of the BP sweep (512^3 cone, S = 2, at most 64 VGPRs):
+// 5 is the most without register spills, 87.6 ms against 89.2 for 6
+// (spills); on another GPU 4 / 6 / 8 took 91.1 / 90.7 / 91.0 ms
 template <int S>
-constexpr int bp_recs() { return S <= 2 ? 6 : (S <= 4 ? 4 : (S <= 8 ? 2 : 1)); }
+constexpr int bp_recs() { return S <= 2 ? 5 : (S <= 4 ? 4 : (S <= 8 ? 2 : 1)); }
 // Trash cells on each side of a BP column array (cells -kBpPad .. -1 and
 // nz .. nz + kBpPad - 1 take the contributions of slices outside the
 // volume); the slice-plane table has the same margin, clamped to planes 0
