@@ -17,7 +17,7 @@ roofline: the dominant projector kernel's algorithmic bytes per launch,
 divided by its mean launch time from HIP events recorded on the launch
 stream inside the timed iterations, against 8 TB/s.  `traffic` is the HBM-
 side bytes per launch from the committed counter summary, reported only
-while the summary's source stamp matches the kernel sources.
+while the summary's source stamp matches the kernel sources' code.
 
 accuracy: 10 CGLS iterations on the same data as the committed FP64 / float
 oracle runs (tests/golden/accuracy_cone512.npz, identical sinogram checked
@@ -36,6 +36,7 @@ import argparse
 import hashlib
 import json
 import os
+import re
 import sys
 import time
 
@@ -57,13 +58,23 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def _code_text(text: str) -> str:
+    """A C++ source without its comments, indentation and blank lines: the
+    part of it that a rebuilt kernel depends on (no string literal in these
+    sources holds a comment marker)."""
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    lines = (" ".join(line.split()) for line in text.splitlines())
+    return "\n".join(line for line in lines if line)
+
+
 def source_stamp(names=KERNEL_SOURCES) -> str:
-    """SHA-256 (first 16 hex digits) of projector kernel sources (by default
-    all of them)."""
+    """SHA-256 (first 16 hex digits) of the code of projector kernel sources
+    (by default all of them), comments and layout left out."""
     h = hashlib.sha256()
     for name in names:
-        with open(os.path.join(ROOT, "ccpi-reconstruction_amd", "csrc", name), "rb") as f:
-            h.update(f.read())
+        with open(os.path.join(ROOT, "ccpi-reconstruction_amd", "csrc", name)) as f:
+            h.update(_code_text(f.read()).encode())
     return h.hexdigest()[:16]
 
 

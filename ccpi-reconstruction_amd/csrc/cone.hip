@@ -720,19 +720,20 @@ __device__ inline BpSlot bp_slot(const ConeArgs &g, int v, int pass, bool one_se
 // fall on distinct slices, and so do their second slices; slot t has its own
 // LDS array, so the slots' first-slice updates are plain read-add-writes
 // over distinct cells, issued together, followed by their second-slice
-// updates.  The kBpPad cells on each side
-// of an array are trash for slices outside the volume.  PHASED (rows per
-// slice beyond 4, or arrays beyond the LDS budget): S = 4, 8, 12 or 16 slots
-// and one array; the slots update one after another, each a read-add-write
-// round, and within a slot the lanes go in phases (lane L in phase L mod
-// phases), whose lanes are phases * S >= rows per slice rows apart, so on
-// distinct slices.  The phases are the column's own (bp_col_phases), at
-// most the layout's g.bp_phases.  (LDS float atomics on one array made the
-// kernel twice as slow as these rounds, and rows interleaved over passes
-// re-read each record's detector row once per pass.)
+// updates.  The kBpPad cells on each side of an array are trash for slices
+// outside the volume.  PHASED (rows per slice beyond 4, or arrays beyond
+// the LDS budget): S = 4, 8, 12 or 16 slots and one array; the slots update
+// one after another, each in read-add-write rounds, and within a slot the
+// lanes go in phases (lane L in phase L mod phases), whose lanes are
+// phases * S >= rows per slice rows apart, so on distinct slices.  The
+// phases are the column's own (bp_col_phases), at most the layout's
+// g.bp_phases.  (LDS float atomics, ds_add_f32, made the kernel 2 to 19
+// times slower than these rounds, and rows interleaved over passes re-read
+// each record's detector row once per pass.)
 // Waves per SIMD the register allocation must allow: the extra waves hide
-// the LDS read-add-write latency, and the few spills they force sit in the
-// per-row-block set-up.  Strides 1 and 2: 8 (S = 2 at 512^3: 98 -> 93 ms).
+// the LDS read-add-write latency; for the phased strides the few spills
+// they force sit in the per-row-block set-up.  Strides 1 and 2: 8 (S = 2
+// at 512^3: 98 -> 93 ms against 6).
 // Phased, measured on wide cones (ms per BP, waves per SIMD): S = 8 at
 // 384^3 32.2 (6) / 29.8 (7) / 29.1 (8); S = 12 at 768^3 442.6 (5) / 403.4 (6)
 // / 422.7 (7) / 602.0 (8); S = 16 at 1000^3 576.2 (4) / 503.0 (5) / 868.0
