@@ -13,6 +13,10 @@ A12 = np.linspace(0, 2 * np.pi, 12, endpoint=False) + 0.1
 CASES = {
     "s1": (CB((10, 10, 8), (1, 1, 1), np.linspace(0, 2 * np.pi, 12, endpoint=False) + 0.1,
               centred(20, 1.6), centred(8, 3.0), 40.0, 80.0), (1, False, 1)),
+    # three 128-row blocks: one under the source row, one across it, one
+    # above it with padded rows; floor-form and main rows share the first
+    "s2_blocks": (CB((10, 10, 120), (1, 1, 1), A12, centred(20, 1.6), centred(300, 1.6),
+                     40.0, 80.0), (2, False, 1)),
     "s3": (CB((10, 10, 6), (1, 1, 1), np.linspace(0, 2 * np.pi, 12, endpoint=False) + 0.1,
               centred(20, 1.6), centred(20, 0.95), 40.0, 80.0), (3, False, 1)),
     "s4": (CB((10, 10, 6), (1, 1, 1), np.linspace(0, 2 * np.pi, 12, endpoint=False) + 0.1,
@@ -65,7 +69,7 @@ def bp_path(g):
     amin = (math.hypot(g.source[0], g.source[1]) - R) / Lmax
     want = max(1, math.ceil(min(vs[2] / (amin * np.diff(g.v_pix).min()) * 1.05, 1e6)))
     s = min(16, want)
-    pad = (nz + 2 + 3) & ~3
+    pad = (nz + 8 + 3) & ~3  # four trash cells on each side
     lds = 4 * (pad + 4 * s * pad)
     phased = not (want <= 16 and s <= 4 and lds <= 64 * 1024)
     if not phased:

@@ -8,7 +8,10 @@ written slot after slot, and within a slot the lanes in ceil(want / S)
 phases (past 4 rows per slice, S = 8, 12 or 16 by the least padded rows x
 (1 + 0.11 per extra phase)); each column runs its own phases, from the
 smallest walk parameter of its rays, at most the layout's.  These
-geometries select S = 1, 2 (the default test cases), 3, 4, and the phased
+geometries select S = 1, 2 (the default test cases), 3, 4, and S = 2 over
+three 128-row blocks (one under the source row, one across it, one above it
+with padded rows, so the row-set addresses of both directions and of the
+mixed block run, with floor-form and main rows in one block), and the phased
 form with:
   * S = 8: fewer than 64 rows, with the source at mid-height and near the
     volume's top; a padded block of main rows (every ray inside the
