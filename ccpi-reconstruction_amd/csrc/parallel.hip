@@ -63,13 +63,15 @@ __global__ __launch_bounds__(256) void par_bp_kernel(ParArgs g,
 // The tiled BP as one launch: blockIdx.x = tile, y = slice chunk, z = angle
 // group; one group writes vol (adding to it when acc_in), several write
 // their partial sums to part[group] for par_bp_sum_kernel.  norm (one
-// group only): one partial of ||vol||^2 per workgroup.
+// group only): one partial of ||vol||^2 per workgroup.  The body waits on
+// its staged rows and footprints, not on issue: 5 waves per SIMD (94
+// VGPRs, a few spills in the set-up) instead of the 4 that 103 VGPRs allow
+// took the 64-problem mix's BP from 21.9 to 18.8 ms per round (6 waves:
+// 19.2), the 256 x 256 x 16 BP unchanged.
 template <int R, class Acc>
-__global__ __launch_bounds__(256) void par_bp_lds_kernel(ParArgs g, int hmax, int ab, int groups,
-                                                         const float *__restrict__ sino,
-                                                         float *__restrict__ vol,
-                                                         float *__restrict__ part, int acc_in,
-                                                         double *__restrict__ norm) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void par_bp_lds_kernel(
+    ParArgs g, int hmax, int ab, int groups, const float *__restrict__ sino,
+    float *__restrict__ vol, float *__restrict__ part, int acc_in, double *__restrict__ norm) {
   extern __shared__ float smem[];
   const int grp = blockIdx.z;
   const int a_begin = (int)((long long)g.na * grp / groups);
