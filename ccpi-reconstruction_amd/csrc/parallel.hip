@@ -25,6 +25,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -67,9 +68,14 @@ __global__ __launch_bounds__(256) void par_bp_kernel(ParArgs g,
 // its staged rows and footprints, not on issue: 5 waves per SIMD (94
 // VGPRs, a few spills in the set-up) instead of the 4 that 103 VGPRs allow
 // took the 64-problem mix's BP from 21.9 to 18.8 ms per round (6 waves:
-// 19.2), the 256 x 256 x 16 BP unchanged.
+// 19.2), the 256 x 256 x 16 BP unchanged.  The FP64-accumulation instances
+// keep their registers.
+template <class Acc>
+constexpr int par_bp_lds_min_waves() { return std::is_same<Acc, float>::value ? 5 : 1; }
+
 template <int R, class Acc>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void par_bp_lds_kernel(
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(par_bp_lds_min_waves<Acc>(), 8))) void par_bp_lds_kernel(
     ParArgs g, int hmax, int ab, int groups, const float *__restrict__ sino,
     float *__restrict__ vol, float *__restrict__ part, int acc_in, double *__restrict__ norm) {
   extern __shared__ float smem[];
