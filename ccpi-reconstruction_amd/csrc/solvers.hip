@@ -557,21 +557,19 @@ void run_sirt(const Projector &P, const float *b, float *x, int iters, Workspace
   float *C = W.v1.as<float>(), *t = W.v2.as<float>(), *ones = W.v3.as<float>();
   float *R = W.p1.as<float>(), *r = W.p2.as<float>();
   // R = 1 / (A 1), C = 1 / (A^T 1) (sirt.cpp:31-48)
+  // the projections write over their outputs (the reference adds to
+  // zeroed arrays: the same sums)
   dev_fill(ones, nvox, 1.f, st);
-  dev_fill(R, npix, 0.f, st);
-  P.fp(ones, R, st);
+  P.fp(ones, R, st, false, false);
   hipLaunchKernelGGL(invert_min_kernel, dim3(grid_for(npix)), dim3(256), 0, st, R, npix);
   dev_fill(r, npix, 1.f, st);
-  dev_fill(C, nvox, 0.f, st);
-  P.bp(r, C, st, W.bp_scratch());
+  P.bp(r, C, st, W.bp_scratch(), false, false);
   hipLaunchKernelGGL(invert_min_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, C, nvox);
   for (int it = 0; it < iters; it++) {
-    dev_fill(r, npix, 0.f, st);
-    P.fp(x, r, st);
+    P.fp(x, r, st, false, false);
     hipLaunchKernelGGL(sirt_residual_kernel, dim3(grid_for(npix)), dim3(256), 0, st, r, b, R,
                        npix);
-    dev_fill(t, nvox, 0.f, st);
-    P.bp(r, t, st, W.bp_scratch());
+    P.bp(r, t, st, W.bp_scratch(), false, false);
     hipLaunchKernelGGL(sirt_update_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, x, t, C,
                        nvox);
   }
@@ -588,17 +586,16 @@ void run_mlem(const Projector &P, const float *b, float *x, int iters, Workspace
   float *N = W.v1.as<float>(), *t = W.v2.as<float>();
   float *r = W.p1.as<float>();
   // x = 1; N = 1 / (A^T 1) keeping zeros (mlem.cpp:30-41)
+  // the projections write over their outputs (the reference adds to
+  // zeroed arrays: the same sums)
   dev_fill(x, nvox, 1.f, st);
   dev_fill(r, npix, 1.f, st);
-  dev_fill(N, nvox, 0.f, st);
-  P.bp(r, N, st, W.bp_scratch(), f64);
+  P.bp(r, N, st, W.bp_scratch(), f64, false);
   hipLaunchKernelGGL(invert_nz_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, N, nvox);
   for (int it = 0; it < iters; it++) {
-    dev_fill(r, npix, 0.f, st);
-    P.fp(x, r, st, f64);
+    P.fp(x, r, st, f64, false);
     hipLaunchKernelGGL(mlem_ratio_kernel, dim3(grid_for(npix)), dim3(256), 0, st, r, b, npix);
-    dev_fill(t, nvox, 0.f, st);
-    P.bp(r, t, st, W.bp_scratch(), f64);
+    P.bp(r, t, st, W.bp_scratch(), f64, false);
     hipLaunchKernelGGL(mlem_update_kernel, dim3(grid_for(nvox)), dim3(256), 0, st, x, N, t,
                        nvox);
   }
