@@ -1172,18 +1172,17 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
 }
 
 // dpix[r][v] = pix[r][v] * d_conv[h][v] for v < nv, 0 up to the padded row
-// stride nvp (c2D.cpp:1105-1114).
+// stride nvp (c2D.cpp:1105-1114).  A workgroup takes whole rays, its
+// threads the rows.
 __global__ void cone_scale_kernel(const float *__restrict__ pix,
                                   const float *__restrict__ dconv,
                                   float *__restrict__ dpix, size_t nrays, int nh, int nv,
                                   int nvp) {
-  const size_t total = nrays * (size_t)nvp;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
-    const size_t r = q / (size_t)nvp;
-    const int v = (int)(q - r * (size_t)nvp);
-    const int h = (int)(r % (size_t)nh);
-    dpix[q] = v < nv ? pix[r * (size_t)nv + v] * dconv[(size_t)h * nv + v] : 0.0f;
+  for (size_t r = blockIdx.x; r < nrays; r += gridDim.x) {
+    const float *__restrict__ dc = dconv + (r % (size_t)nh) * (size_t)nv;
+    const float *__restrict__ pr = pix + r * (size_t)nv;
+    float *__restrict__ dp = dpix + r * (size_t)nvp;
+    for (int v = threadIdx.x; v < nvp; v += blockDim.x) dp[v] = v < nv ? pr[v] * dc[v] : 0.0f;
   }
 }
 
@@ -1510,8 +1509,7 @@ size_t cone_dpix_floats(const ConePlan &P) {
 void cone_scale(const ConePlan &P, const float *pix, float *dpix, hipStream_t st) {
   const ConeArgs &A = P.args;
   const size_t nrays = (size_t)A.na * A.nh;
-  const size_t total = nrays * (size_t)A.nvp;
-  const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
+  const int blocks = (int)std::min<size_t>(nrays, 16384);
   hipLaunchKernelGGL(cone_scale_kernel, dim3(blocks), dim3(256), 0, st, pix, A.dconv, dpix,
                      nrays, A.nh, A.nv, A.nvp);
   CCPI_HIP(hipGetLastError());

@@ -206,7 +206,8 @@ __global__ void cgls_update_xr_kernel(float *__restrict__ x, const float *__rest
 // d, r -= alpha Ad, and dpix = r * d_conv in the BP's padded row layout
 // (the reference's scaled copy, c2D.cpp:1105-1114; cone_scale_kernel's
 // float product), so the BP of the new residual needs no scaling pass.
-// Grid-stride over the dpix elements (nrays * nvp) and the voxels.
+// Grid-stride over the voxels, then over whole rays (a workgroup's threads
+// over a ray's rows: no per-element division for the row and ray).
 __global__ void cgls_update_xr_scale_kernel(float *__restrict__ x, const float *__restrict__ d,
                                             size_t nvox, float *__restrict__ r,
                                             const float *__restrict__ Ad, size_t nrays, int nh,
@@ -217,18 +218,20 @@ __global__ void cgls_update_xr_scale_kernel(float *__restrict__ x, const float *
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (size_t q = t0; q < nvox; q += stride) x[q] = x[q] + a * d[q];
-  const size_t total = nrays * (size_t)nvp;
-  for (size_t q = t0; q < total; q += stride) {
-    const size_t ray = q / (size_t)nvp;
-    const int v = (int)(q - ray * (size_t)nvp);
-    float out = 0.0f;
-    if (v < nv) {
-      const size_t i = ray * (size_t)nv + v;
-      const float rn = r[i] + (-a) * Ad[i];
-      r[i] = rn;
-      out = rn * dconv[(size_t)(ray % (size_t)nh) * nv + v];
+  for (size_t ray = blockIdx.x; ray < nrays; ray += gridDim.x) {
+    const float *__restrict__ dc = dconv + (ray % (size_t)nh) * (size_t)nv;
+    float *__restrict__ rr = r + ray * (size_t)nv;
+    const float *__restrict__ ar = Ad + ray * (size_t)nv;
+    float *__restrict__ dp = dpix + ray * (size_t)nvp;
+    for (int v = threadIdx.x; v < nvp; v += blockDim.x) {
+      float out = 0.0f;
+      if (v < nv) {
+        const float rn = rr[v] + (-a) * ar[v];
+        rr[v] = rn;
+        out = rn * dc[v];
+      }
+      dp[v] = out;
     }
-    dpix[q] = out;
   }
 }
 
