@@ -1173,16 +1173,31 @@ __attribute__((amdgpu_waves_per_eu(bp_min_waves(S), 8))) void cone_bp_kernel(
 
 // dpix[r][v] = pix[r][v] * d_conv[h][v] for v < nv, 0 up to the padded row
 // stride nvp (c2D.cpp:1105-1114).  A workgroup takes whole rays, its
-// threads the rows.
+// threads the rows, 16 bytes at a time where the rows allow it (dconv and
+// dpix are the plan's and a workspace's: 256-byte aligned).
 __global__ void cone_scale_kernel(const float *__restrict__ pix,
                                   const float *__restrict__ dconv,
                                   float *__restrict__ dpix, size_t nrays, int nh, int nv,
                                   int nvp) {
+  const bool vec = ((nv & 3) | (reinterpret_cast<uintptr_t>(pix) & 15)) == 0;
   for (size_t r = blockIdx.x; r < nrays; r += gridDim.x) {
     const float *__restrict__ dc = dconv + (r % (size_t)nh) * (size_t)nv;
     const float *__restrict__ pr = pix + r * (size_t)nv;
     float *__restrict__ dp = dpix + r * (size_t)nvp;
-    for (int v = threadIdx.x; v < nvp; v += blockDim.x) dp[v] = v < nv ? pr[v] * dc[v] : 0.0f;
+    if (vec) {
+      const int nv4 = nv >> 2, nvp4 = nvp >> 2;
+      for (int v = threadIdx.x; v < nvp4; v += blockDim.x) {
+        float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (v < nv4) {
+          const float4 pv = reinterpret_cast<const float4 *>(pr)[v];
+          const float4 cv = reinterpret_cast<const float4 *>(dc)[v];
+          out = make_float4(pv.x * cv.x, pv.y * cv.y, pv.z * cv.z, pv.w * cv.w);
+        }
+        reinterpret_cast<float4 *>(dp)[v] = out;
+      }
+    } else {
+      for (int v = threadIdx.x; v < nvp; v += blockDim.x) dp[v] = v < nv ? pr[v] * dc[v] : 0.0f;
+    }
   }
 }
 
