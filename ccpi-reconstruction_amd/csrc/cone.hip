@@ -733,7 +733,7 @@ __device__ inline BpSlot bp_slot(const ConeArgs &g, int v, int pass, bool one_se
 // Waves per SIMD the register allocation must allow: the extra waves hide
 // the LDS read-add-write latency; for the phased strides the few spills
 // they force sit in the per-row-block set-up.  Strides 1 and 2: 8 (S = 2
-// at 512^3: 98 -> 93 ms against 6).
+// at 512^3: 98 ms at 6 waves, 93 at 8).
 // Phased, measured on wide cones (ms per BP, waves per SIMD): S = 8 at
 // 384^3 32.2 (6) / 29.8 (7) / 29.1 (8); S = 12 at 768^3 442.6 (5) / 403.4 (6)
 // / 422.7 (7) / 602.0 (8); S = 16 at 1000^3 576.2 (4) / 503.0 (5) / 868.0
